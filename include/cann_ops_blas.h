@@ -1,0 +1,128 @@
+/*
+ * aclblas C API — Level-3 GEMM and batched-LU hot path, MI355X (gfx950) build.
+ *
+ * Every entry point keeps the reference's name, argument order, argument
+ * meaning, status codes and host-pointer alpha/beta semantics, so a program
+ * linked against the reference's libops_blas.so links against this library
+ * unchanged.  Each declaration cites the reference prototype it replaces.
+ *
+ * Conventions shared by every call:
+ *  - Matrices are column-major.  Device pointers for matrices, device arrays of
+ *    device pointers for the batched LAPACK calls, host pointers for alpha/beta
+ *    (read once, on the host, during the call).
+ *  - Work is enqueued asynchronously on the handle's stream (aclblasSetStream);
+ *    results are visible after the caller synchronises that stream.  No call on
+ *    the hot path allocates device memory or synchronises, so sequences of them
+ *    can be captured into a HIP graph.
+ *  - A handle is not thread-safe: it owns one workspace.
+ */
+#ifndef ACLBLAS_AMD_CANN_OPS_BLAS_H
+#define ACLBLAS_AMD_CANN_OPS_BLAS_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <acl/acl.h>
+#include "cann_ops_blas_common.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct _aclblas_handle* aclblasHandle_t;
+typedef void (*aclblasLogCallback)(const char*);
+
+/* ------------------------------------------------------------------------ */
+/* Handle, stream, workspace, version, logging                               */
+/* ------------------------------------------------------------------------ */
+
+/* reference include/cann_ops_blas.h:41.  *handle must be NULL on entry
+ * (INVALID_VALUE otherwise).  Pre-allocates a 32 MiB library workspace on the
+ * current HIP device; ALLOC_FAILED if that allocation fails.  On a host with no
+ * visible GPU the handle is created without a workspace so that argument
+ * checking can be exercised; any call that would launch work then returns
+ * EXECUTION_FAILED. */
+aclblasStatus_t aclblasCreate(aclblasHandle_t* handle);
+
+/* reference include/cann_ops_blas.h:53.  Synchronises the handle's stream,
+ * frees the library workspace (never a user workspace), frees the handle. */
+aclblasStatus_t aclblasDestroy(aclblasHandle_t handle);
+
+/* reference include/cann_ops_blas.h:66.  Synchronises the old stream, binds
+ * the new one (NULL = default stream) and reverts to the library workspace. */
+aclblasStatus_t aclblasSetStream(aclblasHandle_t handle, aclrtStream stream);
+
+/* reference include/cann_ops_blas.h:76 */
+aclblasStatus_t aclblasGetStream(aclblasHandle_t handle, aclrtStream* stream);
+
+/* reference include/cann_ops_blas.h:93.  Borrows caller device memory; grow
+ * only (a smaller user workspace than the current one is ignored). */
+aclblasStatus_t aclblasSetWorkspace(aclblasHandle_t handle, void* workspace, size_t workspaceSize);
+
+/* reference include/cann_ops_blas.h:105.  MAJOR*10000 + MINOR*100 + PATCH. */
+aclblasStatus_t aclblasGetVersion(aclblasHandle_t handle, int* version);
+
+/* reference include/cann_ops_blas.h:116-133 (declared there, defined here with
+ * C linkage).  Messages are emitted only on the host, never on a launch path. */
+aclblasStatus_t aclblasLoggerConfigure(int logIsOn, int logToStdOut, int logToStdErr, const char* logFile);
+aclblasStatus_t aclblasSetLoggerCallback(aclblasLogCallback userCallback);
+aclblasStatus_t aclblasGetLoggerCallback(aclblasLogCallback* userCallback);
+
+/* ------------------------------------------------------------------------ */
+/* Level-3 GEMM                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* C = alpha*op(A)*op(B) + beta*C, FP32.  Same semantics as aclblasSgemmEx with
+ * algo = ACLBLAS_GEMM_DEFAULT (reference blas/gemm_ex/arch35/sgemm_ex_host.cpp:404);
+ * the reference's header has no plain Sgemm, the signature is cuBLAS's. */
+aclblasStatus_t aclblasSgemm(
+    aclblasHandle_t handle, aclblasOperation_t transA, aclblasOperation_t transB, int m, int n, int k,
+    const float* alpha, const float* A, int lda, const float* B, int ldb,
+    const float* beta, float* C, int ldc);
+
+/* reference include/cann_ops_blas.h:443-446 */
+aclblasStatus_t aclblasSgemmEx(
+    aclblasHandle_t handle, aclblasOperation_t transA, aclblasOperation_t transB, int m, int n, int k,
+    const float* alpha, const float* A, int lda, const float* B, int ldb,
+    const float* beta, float* C, int ldc, aclblasGemmAlgo_t algo);
+
+/* reference include/cann_ops_blas.h:438-441.  Accepted (Atype, Btype, Ctype,
+ * computeType): the reference's table (blas/gemm/arch35/gemm_ex_host.cpp:45-69)
+ * plus BF16xBF16->FP32 and FP16xFP16->FP32 under COMPUTE_32F.  alpha and beta
+ * are host floats for every combination. */
+aclblasStatus_t aclblasGemmEx(
+    aclblasHandle_t handle, aclblasOperation_t transA, aclblasOperation_t transB, int m, int n, int k,
+    const void* alpha, const void* A, aclDataType Atype, int lda, const void* B, aclDataType Btype, int ldb,
+    const void* beta, void* C, aclDataType Ctype, int ldc, aclblasComputeType_t computeType, aclblasGemmAlgo_t algo);
+
+/* ------------------------------------------------------------------------ */
+/* Batched LAPACK (FP32)                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* reference include/cann_ops_blas.h:403-404.  In-place LU with partial
+ * pivoting of batchSize n x n matrices.  Aarray: device array of device
+ * pointers.  PivotArray: device int[n*batchSize] (1-based) or NULL for LU
+ * without pivoting.  infoArray: device int[batchSize]; may be NULL only when
+ * PivotArray is NULL.  info[b] = k+1 for the first zero pivot found at column
+ * k, where factorisation of that matrix stops (pivots after k are not written). */
+aclblasStatus_t aclblasSgetrfBatched(
+    aclblasHandle_t handle, int n, float* const Aarray[], int lda, int* PivotArray, int* infoArray, int batchSize);
+
+/* reference include/cann_ops_blas.h:457-459.  Solves op(A) X = B with the
+ * factors of aclblasSgetrfBatched, overwriting B.  info is a HOST int: 0, or
+ * -i for an illegal i-th argument.  nrhs is limited to [0, 256]. */
+aclblasStatus_t aclblasSgetrsBatched(
+    aclblasHandle_t handle, aclblasOperation_t trans, int n, int nrhs, const float* const Aarray[], int lda,
+    const int* devIpiv, float* const Barray[], int ldb, int* info, int batchCount);
+
+/* reference include/cann_ops_blas.h:430-432.  Out-of-place inverse C = A^-1
+ * from LU factors.  infoArray (device, required): j+1 for the zero diagonal
+ * element u_jj with the largest j, 0 otherwise. */
+aclblasStatus_t aclblasSgetriBatched(
+    aclblasHandle_t handle, int n, const float* const Aarray[], int lda, const int* PivotArray,
+    float* const Carray[], int ldc, int* infoArray, int batchSize);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ACLBLAS_AMD_CANN_OPS_BLAS_H */

@@ -1,0 +1,157 @@
+"""ctypes binding of libops_blas.so, the aclblas C ABI built for MI355X.
+
+The functions mirror the C prototypes of include/cann_ops_blas.h one to one:
+same names, same argument order, pointers passed as integers (device addresses
+from torch tensors' data_ptr(), host scalars as Python floats).  There is no
+fallback: if the library is missing, load() raises.
+
+Layout helpers: a batch of column-major n x n matrices with leading dimension
+lda is a torch tensor of shape (batch, n, lda); element (i, j) of matrix b is
+t[b, j, i].  ptr_array() builds the device array of per-matrix device pointers
+that the batched LAPACK calls take.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libops_blas.so")
+
+# aclblasStatus_t (include/cann_ops_blas_common.h)
+STATUS = {
+    0: "SUCCESS", 1: "NOT_INITIALIZED", 2: "ALLOC_FAILED", 3: "INVALID_VALUE", 4: "MAPPING_ERROR",
+    5: "EXECUTION_FAILED", 6: "INTERNAL_ERROR", 7: "NOT_SUPPORTED", 8: "ARCH_MISMATCH",
+    9: "HANDLE_IS_NULLPTR", 10: "INVALID_ENUM", 11: "UNKNOWN",
+}
+STATUS_CODE = {v: k for k, v in STATUS.items()}
+OP_N, OP_T, OP_C = 111, 112, 113
+OPS = {"N": OP_N, "T": OP_T, "C": OP_C}
+# aclDataType (include/acl/acl.h)
+ACL_FLOAT, ACL_FLOAT16, ACL_BF16, ACL_FLOAT8_E5M2, ACL_FLOAT8_E4M3FN = 0, 1, 27, 35, 36
+DTYPES = {"FP32": ACL_FLOAT, "FP16": ACL_FLOAT16, "BF16": ACL_BF16, "FP8_E5M2": ACL_FLOAT8_E5M2,
+          "FP8_E4M3FN": ACL_FLOAT8_E4M3FN}
+# aclblasComputeType_t
+COMPUTE = {"COMPUTE_16F": 0, "COMPUTE_16F_PEDANTIC": 1, "COMPUTE_32F": 2, "COMPUTE_32F_PEDANTIC": 3,
+           "COMPUTE_32F_FAST_16F": 4, "COMPUTE_32F_FAST_16BF": 5, "COMPUTE_32F_FAST_TF32": 6,
+           "COMPUTE_64F": 7}
+GEMM_DEFAULT = 0
+
+_lib = None
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.POINTER(ctypes.c_float)
+
+# name -> argtypes; every entry is declared in include/cann_ops_blas.h
+_SIGNATURES = {
+    "aclblasCreate": [_P],
+    "aclblasDestroy": [_P],
+    "aclblasSetStream": [_P, _P],
+    "aclblasGetStream": [_P, _P],
+    "aclblasSetWorkspace": [_P, _P, ctypes.c_size_t],
+    "aclblasGetVersion": [_P, _P],
+    "aclblasLoggerConfigure": [_I, _I, _I, ctypes.c_char_p],
+    "aclblasSetLoggerCallback": [_P],
+    "aclblasGetLoggerCallback": [_P],
+    "aclblasSgemm": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I],
+    "aclblasSgemmEx": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _I],
+    "aclblasGemmEx": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _P, _I, _I, _P, _P, _I, _I, _I, _I],
+    "aclblasSgetrfBatched": [_P, _I, _P, _I, _P, _P, _I],
+    "aclblasSgetrsBatched": [_P, _I, _I, _I, _P, _I, _P, _P, _I, _P, _I],
+    "aclblasSgetriBatched": [_P, _I, _P, _I, _P, _P, _I, _P, _I],
+}
+
+
+def load(path=None):
+    """Loads libops_blas.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise RuntimeError("libops_blas.so not found at %s; build it with `make -C ops-blas_amd`" % p)
+        L = ctypes.CDLL(p)
+        for name, args in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)
+
+
+def _fp(x):
+    """Host float scalar (or None) -> pointer argument."""
+    if x is None:
+        return None
+    return ctypes.cast(ctypes.pointer(ctypes.c_float(float(x))), _P)
+
+
+class Handle:
+    """Owns one aclblasHandle_t."""
+
+    def __init__(self):
+        self.h = _P()
+        st = load().aclblasCreate(ctypes.byref(self.h))
+        if st != 0:
+            raise RuntimeError("aclblasCreate failed: %s" % STATUS.get(st, st))
+
+    @property
+    def value(self):
+        return self.h.value
+
+    def set_stream(self, stream_ptr):
+        return load().aclblasSetStream(self.h, _P(stream_ptr))
+
+    def destroy(self):
+        if self.h.value:
+            st = load().aclblasDestroy(self.h)
+            self.h = _P()
+            return st
+        return 0
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+
+# ---------------------------------------------------------------- raw calls
+def sgemm(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    return load().aclblasSgemm(h, ta, tb, m, n, k, _fp(alpha), A, lda, B, ldb, _fp(beta), C, ldc)
+
+
+def sgemm_ex(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, algo=GEMM_DEFAULT):
+    return load().aclblasSgemmEx(h, ta, tb, m, n, k, _fp(alpha), A, lda, B, ldb, _fp(beta), C, ldc, algo)
+
+
+def gemm_ex(h, ta, tb, m, n, k, alpha, A, atype, lda, B, btype, ldb, beta, C, ctype, ldc, compute,
+            algo=GEMM_DEFAULT):
+    return load().aclblasGemmEx(h, ta, tb, m, n, k, _fp(alpha), A, atype, lda, B, btype, ldb, _fp(beta), C,
+                                ctype, ldc, compute, algo)
+
+
+def sgetrf_batched(h, n, Aarray, lda, piv, info, batch):
+    return load().aclblasSgetrfBatched(h, n, Aarray, lda, piv, info, batch)
+
+
+def sgetrs_batched(h, trans, n, nrhs, Aarray, lda, piv, Barray, ldb, info_host, batch):
+    """info_host: a ctypes.c_int (or None); it receives the LAPACK-style argument info."""
+    ip = ctypes.byref(info_host) if info_host is not None else None
+    return load().aclblasSgetrsBatched(h, trans, n, nrhs, Aarray, lda, piv, Barray, ldb, ip, batch)
+
+
+def sgetri_batched(h, n, Aarray, lda, piv, Carray, ldc, info, batch):
+    return load().aclblasSgetriBatched(h, n, Aarray, lda, piv, Carray, ldc, info, batch)
+
+
+# ---------------------------------------------------------------- torch helpers
+def ptr_array(t):
+    """Device int64 tensor holding the address of each t[b] (t: (batch, ...) contiguous CUDA tensor)."""
+    import torch
+    batch = t.shape[0]
+    stride = t.stride(0) * t.element_size()
+    base = t.data_ptr()
+    return torch.tensor([base + b * stride for b in range(batch)], dtype=torch.int64, device=t.device)

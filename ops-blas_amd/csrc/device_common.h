@@ -1,0 +1,84 @@
+// Device-side helpers shared by the batched LAPACK kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+namespace aclblas {
+namespace dev {
+
+constexpr int kWave = 64;
+
+// Partial-pivoting selection key.  The pivot of column k is the candidate row
+// with the largest |a| (NaN counts as 0, so NaN never wins), ties resolved to
+// the lowest current row position; when every candidate is 0 or NaN the row
+// already at position k is kept.  This is LAPACK isamax order applied to the
+// permuted column and agrees with the reference's strict-'>' scan on every
+// tie-free input (reference blas/getrf_batched/arch35/sgetrf_batched_kernel.cpp:52-79).
+//
+// key = bits(|a|) << 32 | (0xFFFF - pos) << 16 | row ; 0 marks "not a candidate".
+// For non-negative floats the IEEE bit pattern orders like the value, so the
+// unsigned maximum of the keys is the pivot.
+__device__ __forceinline__ uint64_t PivotKey(float v, int pos, int row)
+{
+    float a = fabsf(v);
+    uint32_t hi = (a == a) ? __float_as_uint(a) : 0u;
+    uint32_t lo = (static_cast<uint32_t>(0xFFFF - pos) << 16) | static_cast<uint32_t>(row);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ int KeyRow(uint64_t key) { return static_cast<int>(key & 0xFFFFu); }
+__device__ __forceinline__ int KeyPos(uint64_t key) { return 0xFFFF - static_cast<int>((key >> 16) & 0xFFFFu); }
+
+__device__ __forceinline__ uint64_t ShflXor64(uint64_t v, int mask)
+{
+    uint32_t lo = static_cast<uint32_t>(v);
+    uint32_t hi = static_cast<uint32_t>(v >> 32);
+    lo = static_cast<uint32_t>(__shfl_xor(static_cast<int>(lo), mask, kWave));
+    hi = static_cast<uint32_t>(__shfl_xor(static_cast<int>(hi), mask, kWave));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Butterfly max over the 64 lanes of a wave; every lane receives the result.
+__device__ __forceinline__ uint64_t WaveMaxKey(uint64_t k)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        uint64_t o = ShflXor64(k, m);
+        k = o > k ? o : k;
+    }
+    return k;
+}
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [Begin, End).
+// Register arrays indexed by I stay in registers whatever the loop size.
+template <int Begin, int End, typename F>
+__device__ __forceinline__ void StaticFor(F&& f)
+{
+    if constexpr (Begin < End) {
+        f(std::integral_constant<int, Begin>{});
+        StaticFor<Begin + 1, End>(f);
+    }
+}
+
+// Makes a wave-uniform value provably scalar (lives in an SGPR).
+__device__ __forceinline__ int Uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ float ReadLane(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__device__ __forceinline__ int LaneId() { return static_cast<int>(threadIdx.x) & (kWave - 1); }
+
+// Reads the b-th entry of a device array of device pointers.
+template <typename T>
+__device__ __forceinline__ T* BatchPtr(T* const* arr, int b)
+{
+    return arr[b];
+}
+
+}  // namespace dev
+}  // namespace aclblas

@@ -1,0 +1,327 @@
+// aclblasSgemm, aclblasSgemmEx, aclblasGemmEx: Level-3 GEMM on MFMA.
+//
+// Reference: blas/gemm_ex/arch35/sgemm_ex_host.cpp:33-137 (SgemmEx argument
+// checks), :227-265 (k == 0 / alpha == 0 cases), :404-426 (entry: m == 0 or
+// n == 0 returns before any check); blas/gemm/arch35/gemm_ex_host.cpp:45-185
+// (GemmEx dtype table and checks), :295-355 (k == 0 and alpha == 0 cases),
+// :450-494 (entry).
+//
+// MI355X design: one fused kernel per call (gemm_kernel.h).  The reference
+// writes op(A)op(B) to a workspace temporary and runs a second alpha/beta pass
+// whenever alpha != 1 or beta != 0; here alpha and beta are applied in the
+// MFMA kernel's epilogue, so no workspace is needed and C is read at most once.
+#include "gemm_kernel.h"
+#include "internal.h"
+
+namespace aclblas {
+namespace {
+
+using namespace gemm;
+
+// C[:m, :n] = beta * C (beta == 0 writes zeros without reading C).
+template <int OK>
+__global__ __launch_bounds__(256) void ScaleCKernel(void* C, int m, int n, int ldc, float beta)
+{
+    const long long total = static_cast<long long>(m) * n;
+    for (long long e = blockIdx.x * 256ll + threadIdx.x; e < total; e += 256ll * gridDim.x) {
+        const int j = static_cast<int>(e / m);
+        const int i = static_cast<int>(e - static_cast<long long>(j) * m);
+        const size_t idx = static_cast<size_t>(j) * ldc + i;
+        const float v = beta == 0.0f ? 0.0f : beta * LoadC<OK>(C, idx);
+        StoreC<OK>(C, idx, v);
+    }
+}
+
+template <int OK>
+void LaunchScaleC(hipStream_t st, void* C, int m, int n, int ldc, float beta)
+{
+    const long long total = static_cast<long long>(m) * n;
+    long long blocks = (total + 255) / 256;
+    blocks = blocks > 4096 ? 4096 : blocks;
+    hipLaunchKernelGGL((ScaleCKernel<OK>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, C, m, n, ldc,
+                       beta);
+}
+
+void LaunchScale(hipStream_t st, int outKind, void* C, int m, int n, int ldc, float beta)
+{
+    switch (outKind) {
+        case kOutF32: LaunchScaleC<kOutF32>(st, C, m, n, ldc, beta); break;
+        case kOutBF16: LaunchScaleC<kOutBF16>(st, C, m, n, ldc, beta); break;
+        default: LaunchScaleC<kOutF16>(st, C, m, n, ldc, beta); break;
+    }
+}
+
+template <int KA, int KB, int OK>
+void LaunchGemmTyped(hipStream_t st, GemmArgs p)
+{
+    constexpr int EB = KindInfo<KA>::bytes;
+    const bool big = (static_cast<long long>((p.m + 127) / 128) * ((p.n + 127) / 128)) >= 256;
+    (void)EB;
+    if (big) {
+        p.tilesM = (p.m + 127) / 128;
+        p.tilesN = (p.n + 127) / 128;
+        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 128, 128>), dim3(p.tilesM * p.tilesN), dim3(kThreads), 0, st,
+                           p);
+    } else {
+        p.tilesM = (p.m + 63) / 64;
+        p.tilesN = (p.n + 63) / 64;
+        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 64, 64>), dim3(p.tilesM * p.tilesN), dim3(kThreads), 0, st, p);
+    }
+}
+
+bool Aligned16(const void* p, int ld, int eb)
+{
+    return (reinterpret_cast<uintptr_t>(p) % 16 == 0) && ((static_cast<long long>(ld) * eb) % 16 == 0);
+}
+
+int KindOf(aclDataType t)
+{
+    switch (t) {
+        case ACL_FLOAT: return kF32;
+        case ACL_BF16: return kBF16;
+        case ACL_FLOAT16: return kF16;
+        case ACL_FLOAT8_E4M3FN: return kE4M3;
+        case ACL_FLOAT8_E5M2: return kE5M2;
+        default: return -1;
+    }
+}
+
+int OutKindOf(aclDataType t)
+{
+    switch (t) {
+        case ACL_FLOAT: return kOutF32;
+        case ACL_BF16: return kOutBF16;
+        default: return kOutF16;
+    }
+}
+
+int ElemBytes(int kind) { return kind == kF32 ? 4 : (kind == kBF16 || kind == kF16) ? 2 : 1; }
+
+// Launches the fused GEMM for a validated, non-degenerate problem.
+aclblasStatus_t RunGemm(hipStream_t st, int ka, int kb, int ok, bool transA, bool transB, int m, int n, int k,
+                        float alpha, const void* A, int lda, const void* B, int ldb, float beta, void* C, int ldc,
+                        const float* scaleA, const float* scaleB)
+{
+    GemmArgs p{};
+    p.A = A;
+    p.B = B;
+    p.C = C;
+    p.m = m;
+    p.n = n;
+    p.k = k;
+    p.lda = lda;
+    p.ldb = ldb;
+    p.ldc = ldc;
+    p.transA = transA ? 1 : 0;
+    p.transB = transB ? 1 : 0;
+    p.alpha = alpha;
+    p.beta = beta;
+    p.scaleA = scaleA;
+    p.scaleB = scaleB;
+    p.vecA = Aligned16(A, lda, ElemBytes(ka)) ? 1 : 0;
+    p.vecB = Aligned16(B, ldb, ElemBytes(kb)) ? 1 : 0;
+    if (ka == kF32 && kb == kF32 && ok == kOutF32) {
+        LaunchGemmTyped<kF32, kF32, kOutF32>(st, p);
+    } else if (ka == kBF16 && kb == kBF16 && ok == kOutBF16) {
+        LaunchGemmTyped<kBF16, kBF16, kOutBF16>(st, p);
+    } else if (ka == kBF16 && kb == kBF16 && ok == kOutF32) {
+        LaunchGemmTyped<kBF16, kBF16, kOutF32>(st, p);
+    } else if (ka == kF16 && kb == kF16 && ok == kOutF16) {
+        LaunchGemmTyped<kF16, kF16, kOutF16>(st, p);
+    } else if (ka == kF16 && kb == kF16 && ok == kOutF32) {
+        LaunchGemmTyped<kF16, kF16, kOutF32>(st, p);
+    } else if (ka == kE4M3 && kb == kE4M3) {
+        LaunchGemmTyped<kE4M3, kE4M3, kOutF16>(st, p);
+    } else if (ka == kE4M3 && kb == kE5M2) {
+        LaunchGemmTyped<kE4M3, kE5M2, kOutF16>(st, p);
+    } else if (ka == kE5M2 && kb == kE4M3) {
+        LaunchGemmTyped<kE5M2, kE4M3, kOutF16>(st, p);
+    } else if (ka == kE5M2 && kb == kE5M2) {
+        LaunchGemmTyped<kE5M2, kE5M2, kOutF16>(st, p);
+    } else {
+        return ACLBLAS_STATUS_NOT_SUPPORTED;
+    }
+    return LaunchStatus("aclblasGemm");
+}
+
+// ---------------------------------------------------------------- SgemmEx
+aclblasStatus_t ValidateSgemmEx(aclblasHandle_t handle, aclblasOperation_t ta, aclblasOperation_t tb, int m, int n,
+                                int k, int lda, int ldb, int ldc, const float* alpha, const float* beta,
+                                aclblasGemmAlgo_t algo, const float* A, const float* B, const float* C)
+{
+    if (handle == nullptr) {
+        return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    }
+    if (m < 0 || n < 0 || k < 0 || !IsValidOp(ta) || !IsValidOp(tb)) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    const bool tA = ta != ACLBLAS_OP_N;
+    const bool tB = tb != ACLBLAS_OP_N;
+    if (lda < (tA ? MaxInt(1, k) : MaxInt(1, m)) || ldb < (tB ? MaxInt(1, n) : MaxInt(1, k)) ||
+        ldc < MaxInt(1, m)) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (alpha == nullptr || beta == nullptr) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (algo != ACLBLAS_GEMM_DEFAULT) {
+        return ACLBLAS_STATUS_NOT_SUPPORTED;
+    }
+    if (k > 0 && (A == nullptr || B == nullptr)) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (*beta != 0.0f && C == nullptr) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+// ---------------------------------------------------------------- GemmEx
+// Accepted (A, B, C, compute) combinations: the reference's table
+// (gemm_ex_host.cpp:45-69) plus BF16 x BF16 -> FP32 and FP16 x FP16 -> FP32.
+bool ValidDtypes(aclDataType at, aclDataType bt, aclDataType ct, aclblasComputeType_t ctype)
+{
+    const bool fp8a = at == ACL_FLOAT8_E4M3FN || at == ACL_FLOAT8_E5M2;
+    const bool fp8b = bt == ACL_FLOAT8_E4M3FN || bt == ACL_FLOAT8_E5M2;
+    switch (ctype) {
+        case ACLBLAS_COMPUTE_16F:
+            return at == ACL_FLOAT16 && bt == ACL_FLOAT16 && ct == ACL_FLOAT16;
+        case ACLBLAS_COMPUTE_32F:
+            if (at == ACL_FLOAT16) return bt == ACL_FLOAT16 && (ct == ACL_FLOAT16 || ct == ACL_FLOAT);
+            if (at == ACL_BF16) return bt == ACL_BF16 && (ct == ACL_BF16 || ct == ACL_FLOAT);
+            if (at == ACL_FLOAT) return bt == ACL_FLOAT && ct == ACL_FLOAT;
+            if (fp8a) return fp8b && ct == ACL_FLOAT16;
+            return false;
+        default:
+            return false;
+    }
+}
+
+aclblasStatus_t ValidateGemmEx(aclblasHandle_t handle, aclblasOperation_t ta, aclblasOperation_t tb, int m, int n,
+                               int k, int lda, int ldb, int ldc, const void* alpha, const void* beta,
+                               aclDataType at, aclDataType bt, aclDataType ct, aclblasComputeType_t ctype,
+                               aclblasGemmAlgo_t algo, const void* A, const void* B, const void* C)
+{
+    if (handle == nullptr) {
+        return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    }
+    if (m < 0 || n < 0 || k < 0 || !IsValidOp(ta) || !IsValidOp(tb)) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    const bool tA = ta != ACLBLAS_OP_N;
+    const bool tB = tb != ACLBLAS_OP_N;
+    if (lda < (tA ? MaxInt(1, k) : MaxInt(1, m)) || ldb < (tB ? MaxInt(1, n) : MaxInt(1, k)) ||
+        ldc < MaxInt(1, m)) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (alpha == nullptr || beta == nullptr) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (algo != ACLBLAS_GEMM_DEFAULT) {
+        return ACLBLAS_STATUS_NOT_SUPPORTED;
+    }
+    if (!ValidDtypes(at, bt, ct, ctype)) {
+        return ACLBLAS_STATUS_NOT_SUPPORTED;
+    }
+    if (m == 0 || n == 0) {
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    if (k > 0 && (A == nullptr || B == nullptr)) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (C == nullptr && *static_cast<const float*>(beta) != 0.0f) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+}  // namespace
+
+// Internal entry shared with the solve drivers: validated FP32/typed GEMM on a
+// stream, alpha/beta by value.
+aclblasStatus_t GemmLaunch(hipStream_t st, aclDataType at, aclDataType bt, aclDataType ct, bool transA, bool transB,
+                           int m, int n, int k, float alpha, const void* A, int lda, const void* B, int ldb,
+                           float beta, void* C, int ldc, const float* scaleA, const float* scaleB)
+{
+    if (m == 0 || n == 0) {
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    const int ok = OutKindOf(ct);
+    if (k == 0 || alpha == 0.0f) {
+        if (beta != 1.0f) {
+            LaunchScale(st, ok, C, m, n, ldc, beta);
+            return LaunchStatus("aclblasGemm");
+        }
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    return RunGemm(st, KindOf(at), KindOf(bt), ok, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                   scaleA, scaleB);
+}
+
+}  // namespace aclblas
+
+using namespace aclblas;
+
+extern "C" aclblasStatus_t aclblasSgemmEx(
+    aclblasHandle_t handle, aclblasOperation_t transA, aclblasOperation_t transB, int m, int n, int k,
+    const float* alpha, const float* A, int lda, const float* B, int ldb, const float* beta, float* C, int ldc,
+    aclblasGemmAlgo_t algo)
+{
+    // reference sgemm_ex_host.cpp:415-417: quick return before any check
+    if (m == 0 || n == 0) {
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    aclblasStatus_t st =
+        ValidateSgemmEx(handle, transA, transB, m, n, k, lda, ldb, ldc, alpha, beta, algo, A, B, C);
+    if (st != ACLBLAS_STATUS_SUCCESS) {
+        LogError("aclblasSgemmEx", "invalid argument (status %d)", static_cast<int>(st));
+        return st;
+    }
+    const float a = *alpha;
+    const float b = *beta;
+    if (C == nullptr) {
+        return ACLBLAS_STATUS_SUCCESS;  // beta == 0 here (checked above): nothing to write
+    }
+    if ((k == 0 || a == 0.0f) && b == 1.0f) {
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    if (!HandleHasDevice(handle)) {
+        return ACLBLAS_STATUS_EXECUTION_FAILED;
+    }
+    return GemmLaunch(handle->stream, ACL_FLOAT, ACL_FLOAT, ACL_FLOAT, transA != ACLBLAS_OP_N,
+                      transB != ACLBLAS_OP_N, m, n, k, a, A, lda, B, ldb, b, C, ldc, nullptr, nullptr);
+}
+
+extern "C" aclblasStatus_t aclblasSgemm(
+    aclblasHandle_t handle, aclblasOperation_t transA, aclblasOperation_t transB, int m, int n, int k,
+    const float* alpha, const float* A, int lda, const float* B, int ldb, const float* beta, float* C, int ldc)
+{
+    return aclblasSgemmEx(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                          ACLBLAS_GEMM_DEFAULT);
+}
+
+extern "C" aclblasStatus_t aclblasGemmEx(
+    aclblasHandle_t handle, aclblasOperation_t transA, aclblasOperation_t transB, int m, int n, int k,
+    const void* alpha, const void* A, aclDataType Atype, int lda, const void* B, aclDataType Btype, int ldb,
+    const void* beta, void* C, aclDataType Ctype, int ldc, aclblasComputeType_t computeType, aclblasGemmAlgo_t algo)
+{
+    aclblasStatus_t st = ValidateGemmEx(handle, transA, transB, m, n, k, lda, ldb, ldc, alpha, beta, Atype, Btype,
+                                        Ctype, computeType, algo, A, B, C);
+    if (st != ACLBLAS_STATUS_SUCCESS) {
+        LogError("aclblasGemmEx", "invalid argument (status %d)", static_cast<int>(st));
+        return st;
+    }
+    if (m == 0 || n == 0 || C == nullptr) {
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    const float a = *static_cast<const float*>(alpha);
+    const float b = *static_cast<const float*>(beta);
+    if ((k == 0 || a == 0.0f) && b == 1.0f) {
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    if (!HandleHasDevice(handle)) {
+        return ACLBLAS_STATUS_EXECUTION_FAILED;
+    }
+    return GemmLaunch(handle->stream, Atype, Btype, Ctype, transA != ACLBLAS_OP_N, transB != ACLBLAS_OP_N, m, n, k,
+                      a, A, lda, B, ldb, b, C, ldc, nullptr, nullptr);
+}

@@ -1,0 +1,351 @@
+// Handle lifecycle, stream binding, workspace policy, version and logging.
+//
+// Behaviour follows reference blas/common/helper/aclblas_auxiliary.cpp:24-148
+// and aclblas_handle_internal.h:95-209; the logger entry points declared at
+// reference include/cann_ops_blas.h:116-133 are defined here with C linkage.
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+
+#include "internal.h"
+
+namespace aclblas {
+namespace {
+
+struct LoggerState {
+    std::mutex mu;
+    bool on = false;
+    bool to_stdout = false;
+    bool to_stderr = true;
+    const char* file = nullptr;
+    aclblasLogCallback callback = nullptr;
+    int level = 0;  // 0 = errors only when on, 1 = info too
+    bool env_read = false;
+};
+
+LoggerState& Logger()
+{
+    static LoggerState s;
+    return s;
+}
+
+void ReadEnvOnce(LoggerState& s)
+{
+    if (s.env_read) {
+        return;
+    }
+    s.env_read = true;
+    const char* lvl = std::getenv("ACLBLAS_LOG_LEVEL");
+    if (lvl != nullptr && lvl[0] != '\0') {
+        int v = std::atoi(lvl);
+        if (v > 0) {
+            s.on = true;
+            s.level = v > 1 ? 1 : 0;
+        }
+    }
+}
+
+void Emit(int level, const char* api, const char* fmt, va_list ap)
+{
+    LoggerState& s = Logger();
+    std::lock_guard<std::mutex> lock(s.mu);
+    ReadEnvOnce(s);
+    if (!s.on && s.callback == nullptr) {
+        return;
+    }
+    if (level > s.level && s.callback == nullptr) {
+        return;
+    }
+    char body[512];
+    std::vsnprintf(body, sizeof(body), fmt, ap);
+    char line[640];
+    std::snprintf(line, sizeof(line), "[aclblas][%s] %s: %s", level == 0 ? "E" : "I", api, body);
+    if (s.callback != nullptr) {
+        s.callback(line);
+    }
+    if (!s.on) {
+        return;
+    }
+    if (s.to_stdout) {
+        std::fprintf(stdout, "%s\n", line);
+    }
+    if (s.to_stderr) {
+        std::fprintf(stderr, "%s\n", line);
+    }
+    if (s.file != nullptr) {
+        FILE* f = std::fopen(s.file, "a");
+        if (f != nullptr) {
+            std::fprintf(f, "%s\n", line);
+            std::fclose(f);
+        }
+    }
+}
+
+aclblasStatus_t AllocateLibraryWorkspace(_aclblas_handle* h, size_t size)
+{
+    if (size == 0) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, size) != hipSuccess) {
+        (void)hipGetLastError();
+        return ACLBLAS_STATUS_ALLOC_FAILED;
+    }
+    h->workspace = p;
+    h->workspace_size = size;
+    h->library_workspace_size = size;
+    h->owner = WorkspaceOwner::Library;
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+void FreeLibraryWorkspace(_aclblas_handle* h)
+{
+    if (h->owner != WorkspaceOwner::Library || h->workspace == nullptr) {
+        return;
+    }
+    (void)hipFree(h->workspace);
+    h->workspace = nullptr;
+    h->workspace_size = 0;
+}
+
+// Back to a library-owned workspace after a user workspace or a stream change
+// (reference aclblas_handle_internal.h:133-154).
+aclblasStatus_t ResetToLibraryWorkspace(_aclblas_handle* h)
+{
+    if (!HandleHasDevice(h)) {
+        h->workspace = nullptr;
+        h->workspace_size = 0;
+        h->owner = WorkspaceOwner::Library;
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    if (h->owner == WorkspaceOwner::Library && h->workspace != nullptr) {
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    aclblasStatus_t st = SyncHandleStream(h);
+    if (st != ACLBLAS_STATUS_SUCCESS) {
+        return st;
+    }
+    h->workspace = nullptr;
+    h->workspace_size = 0;
+    h->owner = WorkspaceOwner::Library;
+    size_t size = h->library_workspace_size > 0 ? h->library_workspace_size : kDefaultWorkspaceSize;
+    return AllocateLibraryWorkspace(h, size);
+}
+
+}  // namespace
+
+void LogError(const char* api, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    Emit(0, api, fmt, ap);
+    va_end(ap);
+}
+
+void LogInfo(const char* api, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    Emit(1, api, fmt, ap);
+    va_end(ap);
+}
+
+aclblasStatus_t SyncHandleStream(_aclblas_handle* h)
+{
+    if (h == nullptr || !HandleHasDevice(h)) {
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    return hipStreamSynchronize(h->stream) == hipSuccess ? ACLBLAS_STATUS_SUCCESS
+                                                         : ACLBLAS_STATUS_EXECUTION_FAILED;
+}
+
+aclblasStatus_t EnsureWorkspace(_aclblas_handle* h, size_t bytes)
+{
+    if (h == nullptr) {
+        return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    }
+    if (!HandleHasDevice(h)) {
+        return ACLBLAS_STATUS_EXECUTION_FAILED;
+    }
+    if (h->owner == WorkspaceOwner::User) {
+        if (bytes > h->workspace_size) {
+            LogError("aclblasHandle", "user workspace too small: required=%zu available=%zu", bytes,
+                     h->workspace_size);
+            return ACLBLAS_STATUS_ALLOC_FAILED;
+        }
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    if (bytes <= h->workspace_size) {
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    if (bytes > kMaxWorkspaceSize) {
+        LogError("aclblasHandle", "workspace of %zu bytes exceeds the %zu byte limit", bytes, kMaxWorkspaceSize);
+        return ACLBLAS_STATUS_ALLOC_FAILED;
+    }
+    aclblasStatus_t st = SyncHandleStream(h);
+    if (st != ACLBLAS_STATUS_SUCCESS) {
+        return st;
+    }
+    size_t doubled = h->workspace_size > 0 ? h->workspace_size * 2 : kDefaultWorkspaceSize;
+    size_t size = bytes > doubled ? bytes : doubled;
+    if (size > kMaxWorkspaceSize) {
+        size = kMaxWorkspaceSize;
+    }
+    FreeLibraryWorkspace(h);
+    return AllocateLibraryWorkspace(h, size);
+}
+
+}  // namespace aclblas
+
+using namespace aclblas;
+
+extern "C" {
+
+aclblasStatus_t aclblasCreate(aclblasHandle_t* handle)
+{
+    if (handle == nullptr) {
+        return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    }
+    if (*handle != nullptr) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    auto* h = new (std::nothrow) _aclblas_handle();
+    if (h == nullptr) {
+        return ACLBLAS_STATUS_ALLOC_FAILED;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        // No GPU visible: a handle without workspace, usable for argument checks.
+        *handle = h;
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        delete h;
+        return ACLBLAS_STATUS_NOT_INITIALIZED;
+    }
+    h->device = dev;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) {
+        cus = 256;
+    }
+    h->cu_count = cus;
+    aclblasStatus_t st = AllocateLibraryWorkspace(h, kDefaultWorkspaceSize);
+    if (st != ACLBLAS_STATUS_SUCCESS) {
+        delete h;
+        return st;
+    }
+    *handle = h;
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+aclblasStatus_t aclblasDestroy(aclblasHandle_t handle)
+{
+    if (handle == nullptr) {
+        return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    }
+    if (SyncHandleStream(handle) != ACLBLAS_STATUS_SUCCESS) {
+        LogError("aclblasDestroy", "stream synchronisation failed before handle destruction");
+        return ACLBLAS_STATUS_EXECUTION_FAILED;
+    }
+    FreeLibraryWorkspace(handle);
+    delete handle;
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+aclblasStatus_t aclblasSetStream(aclblasHandle_t handle, aclrtStream stream)
+{
+    if (handle == nullptr) {
+        return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    }
+    aclblasStatus_t st = SyncHandleStream(handle);
+    if (st != ACLBLAS_STATUS_SUCCESS) {
+        return st;
+    }
+    handle->stream = reinterpret_cast<hipStream_t>(stream);
+    return ResetToLibraryWorkspace(handle);
+}
+
+aclblasStatus_t aclblasGetStream(aclblasHandle_t handle, aclrtStream* stream)
+{
+    if (handle == nullptr) {
+        return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    }
+    if (stream == nullptr) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    *stream = reinterpret_cast<aclrtStream>(handle->stream);
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+aclblasStatus_t aclblasSetWorkspace(aclblasHandle_t handle, void* workspace, size_t workspaceSize)
+{
+    if (handle == nullptr) {
+        return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    }
+    if (workspace == nullptr || workspaceSize == 0) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (handle->owner == WorkspaceOwner::User && workspaceSize <= handle->workspace_size) {
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    aclblasStatus_t st = SyncHandleStream(handle);
+    if (st != ACLBLAS_STATUS_SUCCESS) {
+        return st;
+    }
+    if (handle->owner == WorkspaceOwner::Library && handle->workspace != nullptr) {
+        handle->library_workspace_size = handle->workspace_size;
+        FreeLibraryWorkspace(handle);
+    }
+    handle->workspace = workspace;
+    handle->workspace_size = workspaceSize;
+    handle->owner = WorkspaceOwner::User;
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+aclblasStatus_t aclblasGetVersion(aclblasHandle_t handle, int* version)
+{
+    (void)handle;
+    if (version == nullptr) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    *version = kVersion;
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+aclblasStatus_t aclblasLoggerConfigure(int logIsOn, int logToStdOut, int logToStdErr, const char* logFile)
+{
+    LoggerState& s = Logger();
+    std::lock_guard<std::mutex> lock(s.mu);
+    s.env_read = true;
+    s.on = logIsOn != 0;
+    s.to_stdout = logToStdOut != 0;
+    s.to_stderr = logToStdErr != 0;
+    s.file = logFile;
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+aclblasStatus_t aclblasSetLoggerCallback(aclblasLogCallback userCallback)
+{
+    LoggerState& s = Logger();
+    std::lock_guard<std::mutex> lock(s.mu);
+    s.callback = userCallback;
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+aclblasStatus_t aclblasGetLoggerCallback(aclblasLogCallback* userCallback)
+{
+    if (userCallback == nullptr) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    LoggerState& s = Logger();
+    std::lock_guard<std::mutex> lock(s.mu);
+    *userCallback = s.callback;
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,78 @@
+// Library-internal declarations shared by the host translation units.
+//
+// The handle keeps the reference's workspace policy (reference
+// blas/common/helper/aclblas_handle_internal.h:26-209): one active workspace
+// that is either library-owned (32 MiB default, grows by doubling up to 2 GiB)
+// or borrowed from the caller (grow-only, never freed by the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+
+#include "cann_ops_blas.h"
+
+namespace aclblas {
+
+constexpr size_t kDefaultWorkspaceSize = 32u * 1024u * 1024u;
+constexpr size_t kMaxWorkspaceSize = 2ull * 1024ull * 1024ull * 1024ull;
+// MAJOR*10000 + MINOR*100 + PATCH (reference blas/common/helper/aclblas_version.h:18-31)
+constexpr int kVersion = 10000;
+// Largest matrix order of the batched LAPACK calls (row indices are packed in
+// 16 bits by the pivot search).
+constexpr int kMaxLapackN = 65535;
+
+enum class WorkspaceOwner { Library, User };
+
+}  // namespace aclblas
+
+struct _aclblas_handle {
+    hipStream_t stream = nullptr;
+    void* workspace = nullptr;
+    size_t workspace_size = 0;
+    aclblas::WorkspaceOwner owner = aclblas::WorkspaceOwner::Library;
+    size_t library_workspace_size = 0;
+    // Device the handle was created on and its compute-unit count (used to size
+    // persistent grids).  device < 0 means no GPU was visible at creation.
+    int device = -1;
+    int cu_count = 0;
+};
+
+namespace aclblas {
+
+// Logging: host-side only, off by default, configured through
+// aclblasLoggerConfigure / aclblasSetLoggerCallback or ACLBLAS_LOG_LEVEL.
+void LogError(const char* api, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void LogInfo(const char* api, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+// Synchronises the handle's stream; EXECUTION_FAILED on error.
+aclblasStatus_t SyncHandleStream(_aclblas_handle* h);
+
+// Grows the library workspace to at least `bytes` (reference
+// aclblas_handle_internal.h:162-209).  Never called on the hot path: every
+// hot-path kernel here runs without a workspace.
+aclblasStatus_t EnsureWorkspace(_aclblas_handle* h, size_t bytes);
+
+// Checks that the handle can launch work and that the last launch was accepted.
+inline bool HandleHasDevice(const _aclblas_handle* h) { return h->device >= 0; }
+
+inline aclblasStatus_t LaunchStatus(const char* api)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        LogError(api, "kernel launch failed: %s", hipGetErrorString(e));
+        return ACLBLAS_STATUS_EXECUTION_FAILED;
+    }
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+inline bool IsValidOp(aclblasOperation_t op)
+{
+    return op == ACLBLAS_OP_N || op == ACLBLAS_OP_T || op == ACLBLAS_OP_C;
+}
+
+inline int MaxInt(int a, int b) { return a > b ? a : b; }
+
+}  // namespace aclblas

@@ -1,0 +1,258 @@
+"""The C ABI on the CPU: exports, handle semantics and every argument-check row.
+
+Runs without a GPU.  libops_blas.so is loaded with ctypes; on a host with no
+visible GPU aclblasCreate returns a handle without workspace, so every status
+code that the reference decides before touching device memory can be checked
+here, row by row from the reference's CSV tables.  Pointers passed in these
+rows are never dereferenced (the calls return during argument checking).
+"""
+import ctypes
+import os
+import re
+
+import pytest
+
+import aclblas as B
+from common import load_cases, status_code
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cann_ops_blas.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"aclblasStatus_t\s+(aclblas\w+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    lib = B.load()
+    declared = _declared()
+    assert declared, "no declarations parsed"
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert sorted(B.exported_symbols()) == declared
+
+
+@pytest.fixture(scope="module")
+def cpu_handle():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            # with a GPU, the handle owns device memory: still fine for argument checks
+            torch.cuda.init()
+    except ImportError:
+        pass
+    h = B.Handle()
+    yield h.value
+    h.destroy()
+
+
+_buf = (ctypes.c_float * 64)()
+DUMMY = ctypes.addressof(_buf)
+
+
+def test_handle_semantics():
+    lib = B.load()
+    assert lib.aclblasCreate(None) == status_code("HANDLE_IS_NULLPTR")
+    h = ctypes.c_void_p(DUMMY)
+    assert lib.aclblasCreate(ctypes.byref(h)) == status_code("INVALID_VALUE")
+    assert lib.aclblasDestroy(None) == status_code("HANDLE_IS_NULLPTR")
+    assert lib.aclblasSetStream(None, None) == status_code("HANDLE_IS_NULLPTR")
+    assert lib.aclblasGetStream(None, None) == status_code("HANDLE_IS_NULLPTR")
+    v = ctypes.c_int(0)
+    assert lib.aclblasGetVersion(None, ctypes.byref(v)) == 0 and v.value == 10000
+    assert lib.aclblasGetVersion(None, None) == status_code("INVALID_VALUE")
+    h = B.Handle()
+    s = ctypes.c_void_p(1)
+    assert lib.aclblasGetStream(h.h, ctypes.byref(s)) == 0 and not s.value
+    assert lib.aclblasGetStream(h.h, None) == status_code("INVALID_VALUE")
+    assert lib.aclblasSetWorkspace(h.h, None, 16) == status_code("INVALID_VALUE")
+    assert lib.aclblasSetWorkspace(h.h, DUMMY, 0) == status_code("INVALID_VALUE")
+    assert h.destroy() == 0
+
+
+def test_logger_entry_points():
+    lib = B.load()
+    assert lib.aclblasLoggerConfigure(0, 0, 0, None) == 0
+    CB = ctypes.CFUNCTYPE(None, ctypes.c_char_p)
+    seen = []
+    cb = CB(lambda msg: seen.append(msg))
+    assert lib.aclblasSetLoggerCallback(ctypes.cast(cb, ctypes.c_void_p)) == 0
+    out = ctypes.c_void_p()
+    assert lib.aclblasGetLoggerCallback(ctypes.byref(out)) == 0
+    assert out.value == ctypes.cast(cb, ctypes.c_void_p).value
+    assert lib.aclblasGetLoggerCallback(None) == status_code("INVALID_VALUE")
+    # an argument error is reported through the callback
+    assert B.sgetrf_batched(None, 4, DUMMY, 4, None, None, 1) == status_code("HANDLE_IS_NULLPTR")
+    lib.aclblasSetLoggerCallback(None)
+
+
+def _bool(v):
+    return str(v).strip().lower() == "true"
+
+
+def _op(s):
+    s = s.replace("ACLBLAS_OP_", "")
+    return B.OPS.get(s, 999)
+
+
+# ------------------------------------------------------------------ LAPACK rows
+def _lapack_rows(name, n_key, batch_key):
+    rows = []
+    for r in load_cases(name):
+        n, batch = int(r[n_key]), int(r[batch_key])
+        if status_code(r["expect_result"]) != 0 or n <= 0 or batch <= 0 or \
+                ("nrhs" in r and int(r["nrhs"]) <= 0):
+            rows.append(r)
+    return rows
+
+
+GETRF_ROWS = _lapack_rows("sgetrf_batched.csv", "n", "batch_size")
+
+
+@pytest.mark.parametrize("row", GETRF_ROWS, ids=[r["case_name"] for r in GETRF_ROWS])
+def test_getrf_argument_rows(cpu_handle, row):
+    kind = row["matrix_type"]
+    h = None if kind == "NULLPTR_HANDLE" else cpu_handle
+    A = None if kind == "NULLPTR_AARRAY" else DUMMY
+    piv = DUMMY if row["pivot_mode"] != "NO_PIVOT" else None
+    info = None if kind == "NULLPTR_INFOARRAY" else DUMMY
+    if status_code(row["expect_result"]) == 0:
+        piv = None  # the reference's no-op path passes no pivot array
+    st = B.sgetrf_batched(h, int(row["n"]), A, int(row["lda"]), piv, info, int(row["batch_size"]))
+    assert st == status_code(row["expect_result"])
+
+
+GETRS_ROWS = _lapack_rows("getrs_batched.csv", "n", "batch_count")
+
+
+def _expected_getrs_info(r):
+    """reference test/getrs_batched/arch35/getrs_batched_test.cpp:76-99"""
+    n, nrhs, batch = int(r["n"]), int(r["nrhs"]), int(r["batch_count"])
+    lda, ldb = int(r["lda"]), int(r["ldb"])
+    kind = r["matrix_type"]
+    if kind == "NULLPTR_HANDLE":
+        return None
+    if _op(r["trans"]) == 999:
+        return -2
+    if n < 0:
+        return -3
+    if nrhs < 0 or nrhs > 256:
+        return -4
+    if n > 0 and nrhs > 0 and batch > 0 and kind == "NULLPTR_AARRAY":
+        return -5
+    if lda < max(1, n):
+        return -6
+    if n > 0 and nrhs > 0 and batch > 0 and kind == "NULLPTR_BARRAY":
+        return -8
+    if ldb < max(1, n):
+        return -9
+    if batch < 0:
+        return -11
+    return 0
+
+
+@pytest.mark.parametrize("row", GETRS_ROWS, ids=[r["case_name"] for r in GETRS_ROWS])
+def test_getrs_argument_rows(cpu_handle, row):
+    kind = row["matrix_type"]
+    h = None if kind == "NULLPTR_HANDLE" else cpu_handle
+    A = None if kind == "NULLPTR_AARRAY" else DUMMY
+    Bp = None if kind == "NULLPTR_BARRAY" else DUMMY
+    info = ctypes.c_int(12345)
+    use_info = row.get("info_mode", "NORMAL") != "NULLPTR"
+    st = B.sgetrs_batched(h, _op(row["trans"]), int(row["n"]), int(row["nrhs"]), A, int(row["lda"]), None, Bp,
+                          int(row["ldb"]), info if use_info else None, int(row["batch_count"]))
+    assert st == status_code(row["expect_result"])
+    if use_info:
+        exp = _expected_getrs_info(row)
+        assert info.value == (12345 if exp is None else exp)
+
+
+GETRI_ROWS = _lapack_rows("sgetri_batched.csv", "n", "batch_size")
+
+
+@pytest.mark.parametrize("row", GETRI_ROWS, ids=[r["case_name"] for r in GETRI_ROWS])
+def test_getri_argument_rows(cpu_handle, row):
+    kind = row["matrix_type"]
+    h = None if kind == "NULLPTR_HANDLE" else cpu_handle
+    A = None if kind == "NULLPTR_AARRAY" else DUMMY
+    C = None if kind == "NULLPTR_CARRAY" else DUMMY
+    info = None if kind == "NULLPTR_INFOARRAY" else DUMMY
+    st = B.sgetri_batched(h, int(row["n"]), A, int(row["lda"]), None, C, int(row["ldc"]), info,
+                          int(row["batch_size"]))
+    assert st == status_code(row["expect_result"])
+
+
+def test_getri_null_handle():
+    assert B.sgetri_batched(None, 4, DUMMY, 4, None, DUMMY, 4, DUMMY, 1) == status_code("HANDLE_IS_NULLPTR")
+
+
+# ------------------------------------------------------------------ GEMM rows
+def _gemm_degenerate(r):
+    m, n = int(r["m"]), int(r["n"])
+    return m == 0 or n == 0
+
+
+SGEMM_ROWS = [r for r in load_cases("sgemm_ex.csv")
+              if status_code(r["expect_result"]) != 0 or _gemm_degenerate(r)]
+
+
+@pytest.mark.parametrize("row", SGEMM_ROWS, ids=[r["case_name"] for r in SGEMM_ROWS])
+def test_sgemm_ex_argument_rows(cpu_handle, row):
+    exp = status_code(row["expect_result"])
+    h = None if exp == status_code("HANDLE_IS_NULLPTR") else cpu_handle
+    alpha = None if _bool(row["alpha_null"]) else float(row["alpha"])
+    beta = None if _bool(row["beta_null"]) else float(row["beta"])
+    A = None if _bool(row["a_null"]) else DUMMY
+    Bp = None if _bool(row["b_null"]) else DUMMY
+    C = None if _bool(row["c_null"]) else DUMMY
+    algo = 0 if row["algo"] == "DEFAULT" else 1 + int(row["algo"].replace("ALGO", ""))
+    args = (h, _op(row["transA"]), _op(row["transB"]), int(row["m"]), int(row["n"]), int(row["k"]), alpha, A,
+            int(row["lda"]), Bp, int(row["ldb"]), beta, C, int(row["ldc"]))
+    assert B.sgemm_ex(*args, algo=algo) == exp
+    if algo == 0:
+        assert B.sgemm(*args) == exp
+
+
+GEMMEX_ROWS = [r for r in load_cases("gemm_ex.csv")
+               if status_code(r["expect_result"]) != 0 or _gemm_degenerate(r)]
+
+
+def gemm_ex_lds(r):
+    """Default leading dimensions of the reference's GemmEx table (blank = minimum)."""
+    m, n, k = int(r["m"]), int(r["n"]), int(r["k"])
+    ta, tb = r["transA"] != "N", r["transB"] != "N"
+    lda = int(r["lda"]) if r["lda"] else max(1, k if ta else m)
+    ldb = int(r["ldb"]) if r["ldb"] else max(1, n if tb else k)
+    ldc = int(r["ldc"]) if r["ldc"] else max(1, m)
+    return lda, ldb, ldc
+
+
+@pytest.mark.parametrize("row", GEMMEX_ROWS, ids=[r["case_name"] for r in GEMMEX_ROWS])
+def test_gemm_ex_argument_rows(cpu_handle, row):
+    exp = status_code(row["expect_result"])
+    h = None if exp == status_code("HANDLE_IS_NULLPTR") else cpu_handle
+    alpha = None if _bool(row["alpha_null"]) else float(row["alpha"])
+    beta = None if _bool(row["beta_null"]) else float(row["beta"])
+    A = None if _bool(row["a_null"]) else DUMMY
+    Bp = None if _bool(row["b_null"]) else DUMMY
+    C = None if _bool(row["c_null"]) else DUMMY
+    lda, ldb, ldc = gemm_ex_lds(row)
+    st = B.gemm_ex(h, _op(row["transA"]), _op(row["transB"]), int(row["m"]), int(row["n"]), int(row["k"]), alpha,
+                   A, B.DTYPES[row["Atype"]], lda, Bp, B.DTYPES[row["Btype"]], ldb, beta, C,
+                   B.DTYPES[row["Ctype"]], ldc, B.COMPUTE[row["computeType"]])
+    assert st == exp
+
+
+def test_gemm_ex_extensions_are_accepted():
+    """BF16 x BF16 -> FP32 and FP16 x FP16 -> FP32 (COMPUTE_32F) pass argument checks."""
+    # m = 0 returns after the dtype check, so a null handle is not needed
+    h = B.Handle()
+    for at, ct in ((B.ACL_BF16, B.ACL_FLOAT), (B.ACL_FLOAT16, B.ACL_FLOAT)):
+        st = B.gemm_ex(h.value, B.OP_N, B.OP_N, 0, 8, 8, 1.0, DUMMY, at, 1, DUMMY, at, 8, 0.0, DUMMY, ct, 1,
+                       B.COMPUTE["COMPUTE_32F"])
+        assert st == 0
+    st = B.gemm_ex(h.value, B.OP_N, B.OP_N, 0, 8, 8, 1.0, DUMMY, B.ACL_BF16, 1, DUMMY, B.ACL_BF16, 8, 0.0, DUMMY,
+                   B.ACL_FLOAT16, 1, B.COMPUTE["COMPUTE_32F"])
+    assert st == status_code("NOT_SUPPORTED")
+    h.destroy()

@@ -1,0 +1,221 @@
+"""GPU parity of aclblasSgetrfBatched / SgetrsBatched / SgetriBatched.
+
+The GPU library is called through its C ABI (ctypes), on the same seeded inputs
+as the oracle.  The bar: pivots and info bit-exact; LU factors, solutions and
+inverses bit-exact against the oracle (the kernels perform the same roundings in
+the same order), and within the reference's MIXED_TOLERANCE (test/frame/
+verify.h:276-333) of LAPACK, the reference's own golden.  Case lists are the
+reference's CSV tables (tests/golden/cases).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.linalg.lapack as lapack
+
+import aclblas as B
+import oracle as O
+from common import load_cases, status_code, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+def _batched_getrf(handle, A, pivot=True, lda=None):
+    """A (batch, n, lda) numpy -> (LU, ipiv, info, status) from the GPU."""
+    import torch
+    batch, n, ld = A.shape
+    dA = to_dev(A)
+    ptrs = B.ptr_array(dA)
+    piv = torch.zeros((batch, n), dtype=torch.int32, device="cuda") if pivot else None
+    info = torch.full((batch,), -1, dtype=torch.int32, device="cuda")
+    st = B.sgetrf_batched(handle.value, n, ptrs.data_ptr(), ld, piv.data_ptr() if pivot else None,
+                          info.data_ptr(), batch)
+    return to_host(dA), (to_host(piv) if pivot else None), to_host(info), st
+
+
+def _matrix_kind(row):
+    return row["matrix_type"]
+
+
+# ------------------------------------------------------------------ getrf CSV
+GETRF_CASES = [r for r in load_cases("sgetrf_batched.csv")]
+
+
+@pytest.mark.parametrize("row", GETRF_CASES, ids=[r["case_name"] for r in GETRF_CASES])
+def test_getrf_reference_cases(handle, row):
+    n, lda, batch = int(row["n"]), int(row["lda"]), int(row["batch_size"])
+    pivot = row["pivot_mode"] != "NO_PIVOT"
+    expect = status_code(row["expect_result"])
+    kind = _matrix_kind(row)
+    seed = int(row["random_seed"])
+    if expect != 0 or n <= 0 or batch <= 0 or kind.startswith("NULLPTR"):
+        pytest.skip("argument-check row: covered by tests/test_abi.py on the CPU and test_status_rows_on_gpu")
+    A = O.lapack_batch(n, lda, kind, seed, batch)
+    LU, ipiv, info, st = _batched_getrf(handle, A, pivot)
+    assert st == expect
+    gLU, gpiv, ginfo = O.getrf_batched(A.copy(), pivot=pivot)
+    assert np.array_equal(info, ginfo), (info, ginfo)
+    if pivot:
+        assert np.array_equal(ipiv, gpiv)
+    ok_rows = info == 0
+    # factors: bit-exact against the oracle
+    assert np.array_equal(LU[ok_rows][:, :, :n], gLU[ok_rows][:, :, :n])
+    # padding rows (lda > n) untouched
+    assert np.array_equal(LU[:, :, n:], A[:, :, n:])
+    # and the reference's own criterion against LAPACK (skipping singular batches)
+    for b in np.nonzero(ok_rows)[0]:
+        if pivot:
+            lu, lpiv, linfo = lapack.sgetrf(A[b, :, :n].T)
+            assert np.array_equal(lpiv + 1, ipiv[b])
+            ok, detail = O.mixed_tolerance(LU[b, :, :n].T, lu)
+            assert ok, detail
+
+
+def test_getrf_c2_config(handle):
+    """4096 x 32x32, N(0,1) + 4I (the solve benchmark's inputs): pivots bit-exact, x within 1e-5 of FP64."""
+    import torch
+    batch, n = 4096, 32
+    A = O.normal((batch, n, n), 0, n=n, batch=batch, shift=4.0)
+    rhs = O.normal((batch, 1, n), 1)
+    LU, ipiv, info, st = _batched_getrf(handle, A)
+    assert st == 0 and np.all(info == 0)
+    gLU, gpiv, _ = O.getrf_batched(A.copy())
+    assert np.array_equal(ipiv, gpiv)
+    assert np.array_equal(LU, gLU)
+    # solve on the GPU with the GPU factors
+    dLU = to_dev(LU)
+    dX = to_dev(rhs)
+    dpiv = to_dev(ipiv)
+    inf = ctypes.c_int(-1)
+    st = B.sgetrs_batched(handle.value, B.OP_N, n, 1, B.ptr_array(dLU).data_ptr(), n, dpiv.data_ptr(),
+                          B.ptr_array(dX).data_ptr(), n, inf, batch)
+    assert st == 0 and inf.value == 0
+    X = to_host(dX)
+    assert np.array_equal(X, O.getrs_batched(0, gLU, gpiv, rhs))
+    exact = np.linalg.solve(A.transpose(0, 2, 1).astype(np.float64), rhs.transpose(0, 2, 1).astype(np.float64))
+    rel = np.abs(X.transpose(0, 2, 1) - exact).max(axis=(1, 2)) / np.abs(exact).max(axis=(1, 2))
+    assert rel.max() <= 1e-5, rel.max()
+    del torch
+
+
+@pytest.mark.parametrize("n", [1, 7, 33, 64, 65, 100, 128, 129, 200, 256, 300])
+def test_getrf_sizes_pivoting(handle, n):
+    """Every kernel path (wave, block, global) against the oracle on pivoting-heavy inputs."""
+    batch = 6 if n <= 256 else 2
+    A = O.normal((batch, n, n), 100 + n)
+    LU, ipiv, info, st = _batched_getrf(handle, A)
+    assert st == 0
+    gLU, gpiv, ginfo = O.getrf_batched(A.copy())
+    assert np.array_equal(info, ginfo)
+    assert np.array_equal(ipiv, gpiv)
+    assert np.array_equal(LU, gLU)
+
+
+def test_getrf_zero_pivot_leaves_later_pivots(handle):
+    import torch
+    n, batch = 40, 4
+    A = O.lapack_batch(n, n, "SINGULAR_ZERO_COL", 42, batch)
+    dA = to_dev(A)
+    piv = torch.full((batch, n), -7, dtype=torch.int32, device="cuda")
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    st = B.sgetrf_batched(handle.value, n, B.ptr_array(dA).data_ptr(), n, piv.data_ptr(), info.data_ptr(), batch)
+    assert st == 0
+    p = to_host(piv)
+    assert to_host(info).tolist() == [1] * batch
+    assert np.all(p[:, 0] == 1) and np.all(p[:, 1:] == -7)
+    assert np.array_equal(to_host(dA), A)
+
+
+# ------------------------------------------------------------------ getrs CSV
+GETRS_CASES = [r for r in load_cases("getrs_batched.csv")]
+
+
+def _factor_with_lapack(A, pivot):
+    """LU factors as the reference's getrs/getri tests make them (LAPACK sgetrf on the CPU)."""
+    batch, n, lda = A.shape
+    LU = A.copy()
+    ipiv = np.zeros((batch, n), np.int32) if pivot else None
+    for b in range(batch):
+        if pivot:
+            lu, p, _ = lapack.sgetrf(A[b, :, :n].T)
+            LU[b, :, :n] = lu.T
+            ipiv[b] = p + 1
+        else:
+            tmp = A[b:b + 1].copy()
+            O.getrf_batched(tmp, pivot=False)
+            LU[b] = tmp[0]
+    return LU, ipiv
+
+
+@pytest.mark.parametrize("row", GETRS_CASES, ids=[r["case_name"] for r in GETRS_CASES])
+def test_getrs_reference_cases(handle, row):
+    n, nrhs = int(row["n"]), int(row["nrhs"])
+    lda, ldb, batch = int(row["lda"]), int(row["ldb"]), int(row["batch_count"])
+    expect = status_code(row["expect_result"])
+    kind = row["matrix_type"]
+    if expect != 0 or n <= 0 or nrhs <= 0 or batch <= 0 or kind.startswith("NULLPTR"):
+        pytest.skip("argument-check row: covered by tests/test_abi.py")
+    pivot = row["pivot_mode"] != "NO_PIVOT"
+    trans = B.OPS[row["trans"].replace("ACLBLAS_OP_", "")]
+    seed = int(row["random_seed"])
+    A = O.lapack_batch(n, lda, kind, seed, batch)
+    LU, ipiv = _factor_with_lapack(A, pivot)
+    Bh = np.stack([O.getrs_rhs(n, nrhs, ldb, seed, b) for b in range(batch)])
+    dLU, dB = to_dev(LU), to_dev(Bh)
+    dpiv = to_dev(ipiv) if pivot else None
+    info = ctypes.c_int(-1)
+    use_info = row.get("info_mode", "NORMAL") != "NULLPTR"
+    st = B.sgetrs_batched(handle.value, trans, n, nrhs, B.ptr_array(dLU).data_ptr(), lda,
+                          dpiv.data_ptr() if pivot else None, B.ptr_array(dB).data_ptr(), ldb,
+                          info if use_info else None, batch)
+    assert st == expect
+    if use_info:
+        assert info.value == 0
+    X = to_host(dB)
+    gX = O.getrs_batched(0 if trans == B.OP_N else 1, LU, ipiv, Bh)
+    assert np.array_equal(X[:, :, :n], gX[:, :, :n], equal_nan=True)
+    assert np.array_equal(X[:, :, n:], Bh[:, :, n:])
+    if n <= 1024:
+        for b in range(batch):
+            ref = lapack.sgetrs(LU[b, :, :n].T, (ipiv[b] - 1) if pivot else np.arange(n),
+                                Bh[b, :, :n].T, trans=0 if trans == B.OP_N else 1)[0]
+            ok, detail = O.mixed_tolerance(X[b, :, :n].T, ref)
+            assert ok or kind.startswith("SINGULAR") or kind == "MIXED", detail
+
+
+# ------------------------------------------------------------------ getri CSV
+GETRI_CASES = [r for r in load_cases("sgetri_batched.csv")]
+
+
+@pytest.mark.parametrize("row", GETRI_CASES, ids=[r["case_name"] for r in GETRI_CASES])
+def test_getri_reference_cases(handle, row):
+    import torch
+    n, lda, ldc, batch = int(row["n"]), int(row["lda"]), int(row["ldc"]), int(row["batch_size"])
+    expect = status_code(row["expect_result"])
+    kind = row["matrix_type"]
+    if expect != 0 or n <= 0 or batch <= 0 or kind.startswith("NULLPTR"):
+        pytest.skip("argument-check row: covered by tests/test_abi.py")
+    pivot = row["pivot_mode"] != "NO_PIVOT"
+    seed = int(row["random_seed"])
+    A = O.lapack_batch(n, lda, kind, seed, batch)
+    LU, ipiv = _factor_with_lapack(A, pivot)
+    dLU = to_dev(LU)
+    C = torch.full((batch, n, ldc), 9.0, dtype=torch.float32, device="cuda")
+    info = torch.full((batch,), -1, dtype=torch.int32, device="cuda")
+    dpiv = to_dev(ipiv) if pivot else None
+    st = B.sgetri_batched(handle.value, n, B.ptr_array(dLU).data_ptr(), lda, dpiv.data_ptr() if pivot else None,
+                          B.ptr_array(C).data_ptr(), ldc, info.data_ptr(), batch)
+    assert st == expect
+    Ch, ih = to_host(C), to_host(info)
+    gC, ginfo = O.getri_batched(LU, ipiv, ldc=ldc)
+    assert np.array_equal(ih, ginfo)
+    assert np.array_equal(Ch[:, :, :n], gC[:, :, :n], equal_nan=True)
+    assert np.all(Ch[:, :, n:] == 9.0)
+    if n <= 512:
+        for b in range(batch):
+            if ginfo[b] != 0:
+                continue
+            ref, linfo = lapack.sgetri(LU[b, :, :n].T, (ipiv[b] - 1) if pivot else np.arange(n))
+            assert linfo == 0
+            ok, detail = O.mixed_tolerance(Ch[b, :, :n].T, ref)
+            assert ok, detail
