@@ -149,9 +149,19 @@ def sgetri_batched(h, n, Aarray, lda, piv, Carray, ldc, info, batch):
 
 # ---------------------------------------------------------------- torch helpers
 def ptr_array(t):
-    """Device int64 tensor holding the address of each t[b] (t: (batch, ...) contiguous CUDA tensor)."""
+    """Device int64 tensor holding the address of each t[b] (t: (batch, ...) contiguous CUDA tensor).
+
+    The pointer array is kept alive as an attribute of `t`: a temporary such as
+    `ptr_array(t).data_ptr()` would otherwise be freed (and its memory handed to
+    the next allocation) before the asynchronous kernel that reads it has run.
+    """
     import torch
     batch = t.shape[0]
     stride = t.stride(0) * t.element_size()
     base = t.data_ptr()
-    return torch.tensor([base + b * stride for b in range(batch)], dtype=torch.int64, device=t.device)
+    cached = getattr(t, "_aclblas_ptrs", None)
+    if cached is not None and cached[0] == (base, stride, batch):
+        return cached[1]
+    ptrs = torch.tensor([base + b * stride for b in range(batch)], dtype=torch.int64, device=t.device)
+    t._aclblas_ptrs = ((base, stride, batch), ptrs)
+    return ptrs

@@ -87,8 +87,9 @@ def test_getrf_c2_config(handle):
     dX = to_dev(rhs)
     dpiv = to_dev(ipiv)
     inf = ctypes.c_int(-1)
-    st = B.sgetrs_batched(handle.value, B.OP_N, n, 1, B.ptr_array(dLU).data_ptr(), n, dpiv.data_ptr(),
-                          B.ptr_array(dX).data_ptr(), n, inf, batch)
+    pa, px = B.ptr_array(dLU), B.ptr_array(dX)
+    st = B.sgetrs_batched(handle.value, B.OP_N, n, 1, pa.data_ptr(), n, dpiv.data_ptr(), px.data_ptr(), n, inf,
+                          batch)
     assert st == 0 and inf.value == 0
     X = to_host(dX)
     assert np.array_equal(X, O.getrs_batched(0, gLU, gpiv, rhs))
@@ -118,7 +119,8 @@ def test_getrf_zero_pivot_leaves_later_pivots(handle):
     dA = to_dev(A)
     piv = torch.full((batch, n), -7, dtype=torch.int32, device="cuda")
     info = torch.zeros(batch, dtype=torch.int32, device="cuda")
-    st = B.sgetrf_batched(handle.value, n, B.ptr_array(dA).data_ptr(), n, piv.data_ptr(), info.data_ptr(), batch)
+    pa = B.ptr_array(dA)
+    st = B.sgetrf_batched(handle.value, n, pa.data_ptr(), n, piv.data_ptr(), info.data_ptr(), batch)
     assert st == 0
     p = to_host(piv)
     assert to_host(info).tolist() == [1] * batch
@@ -165,8 +167,9 @@ def test_getrs_reference_cases(handle, row):
     dpiv = to_dev(ipiv) if pivot else None
     info = ctypes.c_int(-1)
     use_info = row.get("info_mode", "NORMAL") != "NULLPTR"
-    st = B.sgetrs_batched(handle.value, trans, n, nrhs, B.ptr_array(dLU).data_ptr(), lda,
-                          dpiv.data_ptr() if pivot else None, B.ptr_array(dB).data_ptr(), ldb,
+    pa, pb = B.ptr_array(dLU), B.ptr_array(dB)
+    st = B.sgetrs_batched(handle.value, trans, n, nrhs, pa.data_ptr(), lda,
+                          dpiv.data_ptr() if pivot else None, pb.data_ptr(), ldb,
                           info if use_info else None, batch)
     assert st == expect
     if use_info:
@@ -203,8 +206,9 @@ def test_getri_reference_cases(handle, row):
     C = torch.full((batch, n, ldc), 9.0, dtype=torch.float32, device="cuda")
     info = torch.full((batch,), -1, dtype=torch.int32, device="cuda")
     dpiv = to_dev(ipiv) if pivot else None
-    st = B.sgetri_batched(handle.value, n, B.ptr_array(dLU).data_ptr(), lda, dpiv.data_ptr() if pivot else None,
-                          B.ptr_array(C).data_ptr(), ldc, info.data_ptr(), batch)
+    pa, pc = B.ptr_array(dLU), B.ptr_array(C)
+    st = B.sgetri_batched(handle.value, n, pa.data_ptr(), lda, dpiv.data_ptr() if pivot else None,
+                          pc.data_ptr(), ldc, info.data_ptr(), batch)
     assert st == expect
     Ch, ih = to_host(C), to_host(info)
     gC, ginfo = O.getri_batched(LU, ipiv, ldc=ldc)
