@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="before warm-up, run 5 elementwise negations of a MIB-MiB FP32 tensor (read and write "
+                         "MIB MiB each): the known-size reference that scales hardware byte counters")
     return ap.parse_args()
 
 
@@ -98,6 +101,13 @@ def main():
         if st != 0:
             raise RuntimeError("aclblas call failed with status %d" % st)
 
+    if args.calibrate_copy > 0:
+        src = torch.ones(args.calibrate_copy * (1 << 18), device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     for i in range(args.warmup):
         step(i)
     torch.cuda.synchronize()

@@ -94,6 +94,27 @@ aclblasStatus_t aclblasGemmEx(
     const void* alpha, const void* A, aclDataType Atype, int lda, const void* B, aclDataType Btype, int ldb,
     const void* beta, void* C, aclDataType Ctype, int ldc, aclblasComputeType_t computeType, aclblasGemmAlgo_t algo);
 
+/* reference include/cann_ops_blas.h:409-414.  C[i] = alpha*op(A[i])*op(B[i]) +
+ * beta*C[i] over device arrays of device pointers, one shape for the batch.
+ * Accepted types: FP16 (COMPUTE_16F or 32F), BF16 and FP8 E4M3/E5M2 (any mix,
+ * FP16 C) under COMPUTE_32F; FP32 returns NOT_SUPPORTED as in the reference
+ * (use aclblasSgemmBatched). */
+aclblasStatus_t aclblasGemmBatchedEx(
+    aclblasHandle_t handle, aclblasOperation_t transa, aclblasOperation_t transb, int m, int n, int k,
+    const void* alpha, const void* const Aarray[], aclDataType Atype, int lda,
+    const void* const Barray[], aclDataType Btype, int ldb,
+    const void* beta, void* const Carray[], aclDataType Ctype, int ldc,
+    int batchCount, aclblasComputeType_t computeType, aclblasGemmAlgo_t algo);
+
+/* reference include/cann_ops_blas.h:416-420.  FP32 batched GEMM over device
+ * pointer arrays (the MFMA FP32 path). */
+aclblasStatus_t aclblasSgemmBatched(
+    aclblasHandle_t handle, aclblasOperation_t transa, aclblasOperation_t transb, int m, int n, int k,
+    const float* alpha, const float* const Aarray[], int lda,
+    const float* const Barray[], int ldb,
+    const float* beta, float* const Carray[], int ldc,
+    int batchCount);
+
 /* ------------------------------------------------------------------------ */
 /* Batched LAPACK (FP32)                                                     */
 /* ------------------------------------------------------------------------ */

@@ -28,7 +28,7 @@ OPS = {"N": OP_N, "T": OP_T, "C": OP_C}
 # aclDataType (include/acl/acl.h)
 ACL_FLOAT, ACL_FLOAT16, ACL_BF16, ACL_FLOAT8_E5M2, ACL_FLOAT8_E4M3FN = 0, 1, 27, 35, 36
 DTYPES = {"FP32": ACL_FLOAT, "FP16": ACL_FLOAT16, "BF16": ACL_BF16, "FP8_E5M2": ACL_FLOAT8_E5M2,
-          "FP8_E4M3FN": ACL_FLOAT8_E4M3FN}
+          "FP8_E4M3FN": ACL_FLOAT8_E4M3FN, "FP8_E4M3": ACL_FLOAT8_E4M3FN}
 # aclblasComputeType_t
 COMPUTE = {"COMPUTE_16F": 0, "COMPUTE_16F_PEDANTIC": 1, "COMPUTE_32F": 2, "COMPUTE_32F_PEDANTIC": 3,
            "COMPUTE_32F_FAST_16F": 4, "COMPUTE_32F_FAST_16BF": 5, "COMPUTE_32F_FAST_TF32": 6,
@@ -55,6 +55,8 @@ _SIGNATURES = {
     "aclblasSgemm": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I],
     "aclblasSgemmEx": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _I],
     "aclblasGemmEx": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _P, _I, _I, _P, _P, _I, _I, _I, _I],
+    "aclblasGemmBatchedEx": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I],
+    "aclblasSgemmBatched": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _I],
     "aclblasSgetrfBatched": [_P, _I, _P, _I, _P, _P, _I],
     "aclblasSgetrsBatched": [_P, _I, _I, _I, _P, _I, _P, _P, _I, _P, _I],
     "aclblasSgetriBatched": [_P, _I, _P, _I, _P, _P, _I, _P, _I],
@@ -131,6 +133,17 @@ def gemm_ex(h, ta, tb, m, n, k, alpha, A, atype, lda, B, btype, ldb, beta, C, ct
             algo=GEMM_DEFAULT):
     return load().aclblasGemmEx(h, ta, tb, m, n, k, _fp(alpha), A, atype, lda, B, btype, ldb, _fp(beta), C,
                                 ctype, ldc, compute, algo)
+
+
+def gemm_batched_ex(h, ta, tb, m, n, k, alpha, Aarr, atype, lda, Barr, btype, ldb, beta, Carr, ctype, ldc, batch,
+                    compute, algo=GEMM_DEFAULT):
+    return load().aclblasGemmBatchedEx(h, ta, tb, m, n, k, _fp(alpha), Aarr, atype, lda, Barr, btype, ldb, _fp(beta),
+                                       Carr, ctype, ldc, batch, compute, algo)
+
+
+def sgemm_batched(h, ta, tb, m, n, k, alpha, Aarr, lda, Barr, ldb, beta, Carr, ldc, batch):
+    return load().aclblasSgemmBatched(h, ta, tb, m, n, k, _fp(alpha), Aarr, lda, Barr, ldb, _fp(beta), Carr, ldc,
+                                      batch)
 
 
 def sgetrf_batched(h, n, Aarray, lda, piv, info, batch):

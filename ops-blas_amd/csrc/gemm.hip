@@ -51,21 +51,58 @@ void LaunchScale(hipStream_t st, int outKind, void* C, int m, int n, int ldc, fl
     }
 }
 
+// C_b[:m, :n] = beta * C_b for every problem of a batch (pointer array).
+template <int OK>
+__global__ __launch_bounds__(256) void ScaleCBatchedKernel(void* const* Carr, int m, int n, int ldc, float beta)
+{
+    void* C = Carr[blockIdx.y];
+    const long long total = static_cast<long long>(m) * n;
+    for (long long e = blockIdx.x * 256ll + threadIdx.x; e < total; e += 256ll * gridDim.x) {
+        const int j = static_cast<int>(e / m);
+        const int i = static_cast<int>(e - static_cast<long long>(j) * m);
+        const size_t idx = static_cast<size_t>(j) * ldc + i;
+        const float v = beta == 0.0f ? 0.0f : beta * LoadC<OK>(C, idx);
+        StoreC<OK>(C, idx, v);
+    }
+}
+
+void LaunchScaleBatched(hipStream_t st, int outKind, void* const* Carr, int m, int n, int ldc, float beta, int batch)
+{
+    const long long total = static_cast<long long>(m) * n;
+    long long bx = (total + 255) / 256;
+    bx = bx > 64 ? 64 : bx;
+    const dim3 grid(static_cast<unsigned>(bx), static_cast<unsigned>(batch));
+    switch (outKind) {
+        case kOutF32: hipLaunchKernelGGL((ScaleCBatchedKernel<kOutF32>), grid, dim3(256), 0, st, Carr, m, n, ldc, beta); break;
+        case kOutBF16: hipLaunchKernelGGL((ScaleCBatchedKernel<kOutBF16>), grid, dim3(256), 0, st, Carr, m, n, ldc, beta); break;
+        default: hipLaunchKernelGGL((ScaleCBatchedKernel<kOutF16>), grid, dim3(256), 0, st, Carr, m, n, ldc, beta); break;
+    }
+}
+
+// Tile shape per problem: 256 x 256 for large 1- and 2-byte GEMMs (64 KiB LDS
+// per stage), 128 x 128 for large FP32 ones, 64 x 64 when the 128-tile grid
+// would leave compute units idle.
 template <int KA, int KB, int OK>
-void LaunchGemmTyped(hipStream_t st, GemmArgs p)
+void LaunchGemmTyped(hipStream_t st, GemmArgs p, int batch, int cus)
 {
     constexpr int EB = KindInfo<KA>::bytes;
-    const bool big = (static_cast<long long>((p.m + 127) / 128) * ((p.n + 127) / 128)) >= 256;
-    (void)EB;
-    if (big) {
+    const long long t256 = static_cast<long long>((p.m + 255) / 256) * ((p.n + 255) / 256) * batch;
+    const long long t128 = static_cast<long long>((p.m + 127) / 128) * ((p.n + 127) / 128) * batch;
+    if (EB <= 2 && t256 >= cus) {
+        p.tilesM = (p.m + 255) / 256;
+        p.tilesN = (p.n + 255) / 256;
+        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 256, 256>), dim3(p.tilesM * p.tilesN * batch), dim3(kThreads),
+                           0, st, p);
+    } else if (t128 >= cus) {
         p.tilesM = (p.m + 127) / 128;
         p.tilesN = (p.n + 127) / 128;
-        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 128, 128>), dim3(p.tilesM * p.tilesN), dim3(kThreads), 0, st,
-                           p);
+        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 128, 128>), dim3(p.tilesM * p.tilesN * batch), dim3(kThreads),
+                           0, st, p);
     } else {
         p.tilesM = (p.m + 63) / 64;
         p.tilesN = (p.n + 63) / 64;
-        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 64, 64>), dim3(p.tilesM * p.tilesN), dim3(kThreads), 0, st, p);
+        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 64, 64>), dim3(p.tilesM * p.tilesN * batch), dim3(kThreads), 0,
+                           st, p);
     }
 }
 
@@ -97,15 +134,21 @@ int OutKindOf(aclDataType t)
 
 int ElemBytes(int kind) { return kind == kF32 ? 4 : (kind == kBF16 || kind == kF16) ? 2 : 1; }
 
-// Launches the fused GEMM for a validated, non-degenerate problem.
-aclblasStatus_t RunGemm(hipStream_t st, int ka, int kb, int ok, bool transA, bool transB, int m, int n, int k,
-                        float alpha, const void* A, int lda, const void* B, int ldb, float beta, void* C, int ldc,
-                        const float* scaleA, const float* scaleB)
+// Launches the fused GEMM for a validated, non-degenerate problem (one
+// problem when Aarr is null, otherwise `batch` problems through device
+// pointer arrays).
+aclblasStatus_t RunGemm(hipStream_t st, int cus, int ka, int kb, int ok, bool transA, bool transB, int m, int n,
+                        int k, float alpha, const void* A, int lda, const void* B, int ldb, float beta, void* C,
+                        int ldc, const float* scaleA, const float* scaleB, const void* const* Aarr = nullptr,
+                        const void* const* Barr = nullptr, void* const* Carr = nullptr, int batch = 1)
 {
     GemmArgs p{};
     p.A = A;
     p.B = B;
     p.C = C;
+    p.Aarr = Aarr;
+    p.Barr = Barr;
+    p.Carr = Carr;
     p.m = m;
     p.n = n;
     p.k = k;
@@ -118,26 +161,35 @@ aclblasStatus_t RunGemm(hipStream_t st, int ka, int kb, int ok, bool transA, boo
     p.beta = beta;
     p.scaleA = scaleA;
     p.scaleB = scaleB;
-    p.vecA = Aligned16(A, lda, ElemBytes(ka)) ? 1 : 0;
-    p.vecB = Aligned16(B, ldb, ElemBytes(kb)) ? 1 : 0;
+    // alignment of batched pointers is re-checked per problem on the device
+    const void* a0 = Aarr != nullptr ? nullptr : A;
+    const void* b0 = Barr != nullptr ? nullptr : B;
+    const void* c0 = Carr != nullptr ? nullptr : C;
+    p.vecA = Aligned16(a0, lda, ElemBytes(ka)) ? 1 : 0;
+    p.vecB = Aligned16(b0, ldb, ElemBytes(kb)) ? 1 : 0;
+    {
+        const int cb = ok == kOutF32 ? 4 : 2;  // bytes per C element; four move together
+        p.vecC = (reinterpret_cast<uintptr_t>(c0) % (4 * cb) == 0 && (static_cast<long long>(ldc) * cb) % (4 * cb) == 0)
+                     ? 1 : 0;
+    }
     if (ka == kF32 && kb == kF32 && ok == kOutF32) {
-        LaunchGemmTyped<kF32, kF32, kOutF32>(st, p);
+        LaunchGemmTyped<kF32, kF32, kOutF32>(st, p, batch, cus);
     } else if (ka == kBF16 && kb == kBF16 && ok == kOutBF16) {
-        LaunchGemmTyped<kBF16, kBF16, kOutBF16>(st, p);
+        LaunchGemmTyped<kBF16, kBF16, kOutBF16>(st, p, batch, cus);
     } else if (ka == kBF16 && kb == kBF16 && ok == kOutF32) {
-        LaunchGemmTyped<kBF16, kBF16, kOutF32>(st, p);
+        LaunchGemmTyped<kBF16, kBF16, kOutF32>(st, p, batch, cus);
     } else if (ka == kF16 && kb == kF16 && ok == kOutF16) {
-        LaunchGemmTyped<kF16, kF16, kOutF16>(st, p);
+        LaunchGemmTyped<kF16, kF16, kOutF16>(st, p, batch, cus);
     } else if (ka == kF16 && kb == kF16 && ok == kOutF32) {
-        LaunchGemmTyped<kF16, kF16, kOutF32>(st, p);
+        LaunchGemmTyped<kF16, kF16, kOutF32>(st, p, batch, cus);
     } else if (ka == kE4M3 && kb == kE4M3) {
-        LaunchGemmTyped<kE4M3, kE4M3, kOutF16>(st, p);
+        LaunchGemmTyped<kE4M3, kE4M3, kOutF16>(st, p, batch, cus);
     } else if (ka == kE4M3 && kb == kE5M2) {
-        LaunchGemmTyped<kE4M3, kE5M2, kOutF16>(st, p);
+        LaunchGemmTyped<kE4M3, kE5M2, kOutF16>(st, p, batch, cus);
     } else if (ka == kE5M2 && kb == kE4M3) {
-        LaunchGemmTyped<kE5M2, kE4M3, kOutF16>(st, p);
+        LaunchGemmTyped<kE5M2, kE4M3, kOutF16>(st, p, batch, cus);
     } else if (ka == kE5M2 && kb == kE5M2) {
-        LaunchGemmTyped<kE5M2, kE5M2, kOutF16>(st, p);
+        LaunchGemmTyped<kE5M2, kE5M2, kOutF16>(st, p, batch, cus);
     } else {
         return ACLBLAS_STATUS_NOT_SUPPORTED;
     }
@@ -239,9 +291,9 @@ aclblasStatus_t ValidateGemmEx(aclblasHandle_t handle, aclblasOperation_t ta, ac
 
 // Internal entry shared with the solve drivers: validated FP32/typed GEMM on a
 // stream, alpha/beta by value.
-aclblasStatus_t GemmLaunch(hipStream_t st, aclDataType at, aclDataType bt, aclDataType ct, bool transA, bool transB,
-                           int m, int n, int k, float alpha, const void* A, int lda, const void* B, int ldb,
-                           float beta, void* C, int ldc, const float* scaleA, const float* scaleB)
+aclblasStatus_t GemmLaunch(hipStream_t st, int cus, aclDataType at, aclDataType bt, aclDataType ct, bool transA,
+                           bool transB, int m, int n, int k, float alpha, const void* A, int lda, const void* B,
+                           int ldb, float beta, void* C, int ldc, const float* scaleA, const float* scaleB)
 {
     if (m == 0 || n == 0) {
         return ACLBLAS_STATUS_SUCCESS;
@@ -254,8 +306,8 @@ aclblasStatus_t GemmLaunch(hipStream_t st, aclDataType at, aclDataType bt, aclDa
         }
         return ACLBLAS_STATUS_SUCCESS;
     }
-    return RunGemm(st, KindOf(at), KindOf(bt), ok, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
-                   scaleA, scaleB);
+    return RunGemm(st, cus, KindOf(at), KindOf(bt), ok, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                   ldc, scaleA, scaleB);
 }
 
 }  // namespace aclblas
@@ -288,7 +340,7 @@ extern "C" aclblasStatus_t aclblasSgemmEx(
     if (!HandleHasDevice(handle)) {
         return ACLBLAS_STATUS_EXECUTION_FAILED;
     }
-    return GemmLaunch(handle->stream, ACL_FLOAT, ACL_FLOAT, ACL_FLOAT, transA != ACLBLAS_OP_N,
+    return GemmLaunch(handle->stream, handle->cu_count, ACL_FLOAT, ACL_FLOAT, ACL_FLOAT, transA != ACLBLAS_OP_N,
                       transB != ACLBLAS_OP_N, m, n, k, a, A, lda, B, ldb, b, C, ldc, nullptr, nullptr);
 }
 
@@ -322,6 +374,151 @@ extern "C" aclblasStatus_t aclblasGemmEx(
     if (!HandleHasDevice(handle)) {
         return ACLBLAS_STATUS_EXECUTION_FAILED;
     }
-    return GemmLaunch(handle->stream, Atype, Btype, Ctype, transA != ACLBLAS_OP_N, transB != ACLBLAS_OP_N, m, n, k,
+    return GemmLaunch(handle->stream, handle->cu_count, Atype, Btype, Ctype, transA != ACLBLAS_OP_N,
+                      transB != ACLBLAS_OP_N, m, n, k,
                       a, A, lda, B, ldb, b, C, ldc, nullptr, nullptr);
+}
+
+namespace aclblas {
+namespace {
+
+// reference blas/gemm_batched_ex/arch35/gemm_batched_ex_host.cpp:87-118
+bool ValidBatchedDtypes(aclDataType at, aclDataType bt, aclDataType ct, aclblasComputeType_t ctype)
+{
+    const bool fp8a = at == ACL_FLOAT8_E4M3FN || at == ACL_FLOAT8_E5M2;
+    const bool fp8b = bt == ACL_FLOAT8_E4M3FN || bt == ACL_FLOAT8_E5M2;
+    if (at == ACL_FLOAT16 && bt == ACL_FLOAT16 && ct == ACL_FLOAT16) {
+        return ctype == ACLBLAS_COMPUTE_16F || ctype == ACLBLAS_COMPUTE_32F;
+    }
+    if (ctype != ACLBLAS_COMPUTE_32F) {
+        return false;
+    }
+    if (at == ACL_BF16) {
+        return bt == ACL_BF16 && ct == ACL_BF16;
+    }
+    return fp8a && fp8b && ct == ACL_FLOAT16;
+}
+
+// reference gemm_batched_ex_host.cpp:155-244 (order: handle, alpha, beta,
+// dimensions and batch count, leading dimensions, algo, dtypes, FP8 compute
+// type, pointer arrays when batchCount > 0)
+aclblasStatus_t ValidateGemmBatchedEx(aclblasHandle_t handle, aclblasOperation_t ta, aclblasOperation_t tb, int m,
+                                      int n, int k, int lda, int ldb, int ldc, int batch, const void* alpha,
+                                      const void* beta, aclDataType at, aclDataType bt, aclDataType ct,
+                                      aclblasComputeType_t ctype, aclblasGemmAlgo_t algo, const void* A,
+                                      const void* B, const void* C)
+{
+    if (handle == nullptr) {
+        return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    }
+    if (alpha == nullptr || beta == nullptr) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (m < 0 || n < 0 || k < 0 || !IsValidOp(ta) || !IsValidOp(tb) || batch < 0) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    const bool tA = ta != ACLBLAS_OP_N;
+    const bool tB = tb != ACLBLAS_OP_N;
+    if (lda < (tA ? MaxInt(1, k) : MaxInt(1, m)) || ldb < (tB ? MaxInt(1, n) : MaxInt(1, k)) ||
+        ldc < MaxInt(1, m)) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (algo != ACLBLAS_GEMM_DEFAULT) {
+        return ACLBLAS_STATUS_NOT_SUPPORTED;
+    }
+    if (!ValidBatchedDtypes(at, bt, ct, ctype)) {
+        return ACLBLAS_STATUS_NOT_SUPPORTED;
+    }
+    if (batch > 0 && (A == nullptr || B == nullptr || C == nullptr)) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+// reference blas/gemm_batched/arch35/gemm_batched_host.cpp:53-97
+aclblasStatus_t ValidateSgemmBatched(aclblasHandle_t handle, aclblasOperation_t ta, aclblasOperation_t tb, int m,
+                                     int n, int k, int lda, int ldb, int ldc, int batch, const float* alpha,
+                                     const float* beta, const void* A, const void* B, const void* C)
+{
+    if (handle == nullptr) {
+        return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    }
+    if (alpha == nullptr || beta == nullptr) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (m < 0 || n < 0 || k < 0 || batch < 0 || !IsValidOp(ta) || !IsValidOp(tb)) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    const bool tA = ta != ACLBLAS_OP_N;
+    const bool tB = tb != ACLBLAS_OP_N;
+    if (lda < (tA ? MaxInt(1, k) : MaxInt(1, m)) || ldb < (tB ? MaxInt(1, n) : MaxInt(1, k)) ||
+        ldc < MaxInt(1, m)) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (batch > 0 && (A == nullptr || B == nullptr || C == nullptr)) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+aclblasStatus_t RunBatched(aclblasHandle_t handle, aclDataType at, aclDataType bt, aclDataType ct, bool tA, bool tB,
+                           int m, int n, int k, float alpha, const void* const* A, int lda, const void* const* B,
+                           int ldb, float beta, void* const* C, int ldc, int batch)
+{
+    if (!HandleHasDevice(handle)) {
+        return ACLBLAS_STATUS_EXECUTION_FAILED;
+    }
+    const int ok = OutKindOf(ct);
+    if (k == 0 || alpha == 0.0f) {
+        if (beta == 1.0f) {
+            return ACLBLAS_STATUS_SUCCESS;
+        }
+        LaunchScaleBatched(handle->stream, ok, C, m, n, ldc, beta, batch);
+        return LaunchStatus("aclblasGemmBatched");
+    }
+    return RunGemm(handle->stream, handle->cu_count, KindOf(at), KindOf(bt), ok, tA, tB, m, n, k, alpha, nullptr,
+                   lda, nullptr, ldb, beta, nullptr, ldc, nullptr, nullptr, A, B, C, batch);
+}
+
+}  // namespace
+}  // namespace aclblas
+
+extern "C" aclblasStatus_t aclblasGemmBatchedEx(
+    aclblasHandle_t handle, aclblasOperation_t transa, aclblasOperation_t transb, int m, int n, int k,
+    const void* alpha, const void* const Aarray[], aclDataType Atype, int lda, const void* const Barray[],
+    aclDataType Btype, int ldb, const void* beta, void* const Carray[], aclDataType Ctype, int ldc, int batchCount,
+    aclblasComputeType_t computeType, aclblasGemmAlgo_t algo)
+{
+    aclblasStatus_t st = ValidateGemmBatchedEx(handle, transa, transb, m, n, k, lda, ldb, ldc, batchCount, alpha,
+                                               beta, Atype, Btype, Ctype, computeType, algo, Aarray, Barray, Carray);
+    if (st != ACLBLAS_STATUS_SUCCESS) {
+        LogError("aclblasGemmBatchedEx", "invalid argument (status %d)", static_cast<int>(st));
+        return st;
+    }
+    if (m == 0 || n == 0 || batchCount == 0) {
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    return RunBatched(handle, Atype, Btype, Ctype, transa != ACLBLAS_OP_N, transb != ACLBLAS_OP_N, m, n, k,
+                      *static_cast<const float*>(alpha), Aarray, lda, Barray, ldb, *static_cast<const float*>(beta),
+                      Carray, ldc, batchCount);
+}
+
+extern "C" aclblasStatus_t aclblasSgemmBatched(
+    aclblasHandle_t handle, aclblasOperation_t transa, aclblasOperation_t transb, int m, int n, int k,
+    const float* alpha, const float* const Aarray[], int lda, const float* const Barray[], int ldb,
+    const float* beta, float* const Carray[], int ldc, int batchCount)
+{
+    aclblasStatus_t st = ValidateSgemmBatched(handle, transa, transb, m, n, k, lda, ldb, ldc, batchCount, alpha, beta,
+                                              Aarray, Barray, Carray);
+    if (st != ACLBLAS_STATUS_SUCCESS) {
+        LogError("aclblasSgemmBatched", "invalid argument (status %d)", static_cast<int>(st));
+        return st;
+    }
+    if (m == 0 || n == 0 || batchCount == 0) {
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    return RunBatched(handle, ACL_FLOAT, ACL_FLOAT, ACL_FLOAT, transa != ACLBLAS_OP_N, transb != ACLBLAS_OP_N, m, n,
+                      k, *alpha, reinterpret_cast<const void* const*>(Aarray), lda,
+                      reinterpret_cast<const void* const*>(Barray), ldb, *beta,
+                      reinterpret_cast<void* const*>(Carray), ldc, batchCount);
 }

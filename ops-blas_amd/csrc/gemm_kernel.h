@@ -16,10 +16,11 @@
 // MFMA operand mapping (16x16 family): lane l supplies row (l % 16) of the A
 // fragment and column (l % 16) of the B fragment, and reads the 32-byte quarter
 // (l / 16) of its row in the k-block: 8 FP32 values (8 x 16x16x4 MFMAs), 16
-// BF16/FP16 values (2 x 16x16x32) or 32 FP8 values (1 x 16x16x128).  A and B
-// lanes consume the same k indices, so every k is summed exactly once.  The
-// accumulator of lane l holds rows 4*(l/16)..+3 of column l%16: four
-// consecutive elements of one column of C, stored with one vector write.
+// BF16/FP16 values (2 x 16x16x32) or 32 FP8 values (widened to BF16 in
+// registers, 4 x 16x16x32).  A and B lanes consume the same k indices, so every
+// k is summed exactly once.  The accumulator of lane l holds rows
+// 4*(l/16)..+3 of column l%16: four consecutive elements of one column of C,
+// stored with one vector write.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -74,25 +75,40 @@ struct GemmArgs {
     const float* scaleA;    // optional device scalars multiplying op(A), op(B)
     const float* scaleB;
     int vecA, vecB;         // 1 when 16-byte vector loads are legal
+    int vecC;               // 1 when four consecutive C elements can move as one vector
     int tilesM, tilesN;
+    // batched form: device arrays of per-problem pointers (A/B/C unused), or null
+    const void* const* Aarr;
+    const void* const* Barr;
+    void* const* Carr;
 };
 
+// The XOR key mixes row bits 0-2 and 3-5 so that both access patterns are
+// conflict-free: eight consecutive rows at one chunk (MFMA fragment reads) and
+// one chunk of rows eight apart (transposed tile stores).
 __device__ __forceinline__ uint32_t SwizzledOffset(int row, int byteInRow)
 {
-    const int chunk = (byteInRow >> 4) ^ (row & 7);
+    const int chunk = (byteInRow >> 4) ^ ((row ^ (row >> 3)) & 7);
     return static_cast<uint32_t>(row * kRowBytes + (chunk << 4) + (byteInRow & 15));
 }
 
 // Fetches of one operand tile (ROWS rows of op(X), one k-block) from global
 // memory into registers, then into LDS.  Row r of the tile is op(X) row/col
 // index r0 + r; the k-block starts at k0.  kcontig: element (r, kk) sits at
-// X[kk + r*ld] (k contiguous); otherwise at X[r + kk*ld] (rows contiguous).
+// X[kk + r*ld] (k contiguous): each thread moves 16-byte chunks along k.
+// Otherwise element (r, kk) sits at X[r + kk*ld] (rows contiguous): each thread
+// loads KB 16-byte chunks of E rows at KB consecutive k, transposes the E x KB
+// block in registers and stores E row segments of KB elements (2- and 4-byte
+// types; FP8 falls back to element stores).
 template <int ROWS, int EB>
 struct TileLoader {
     static constexpr int kElems = kRowBytes / EB;                 // k per block
     static constexpr int kChunks = ROWS * kRowBytes / 16;         // 16-byte chunks per tile
-    static constexpr int kPerThread = kChunks / kThreads;
+    static constexpr int kPerThread = kChunks / kThreads;         // = KB
+    static constexpr int E = 16 / EB;                             // elements per chunk
+    static constexpr int kMChunks = ROWS / E;                     // row chunks (transposed path)
     static_assert(kChunks % kThreads == 0, "tile must split evenly");
+    static_assert(kMChunks * (kElems / kPerThread) == kThreads, "transposed blocks cover the tile");
     uint4 v[kPerThread];
 
     __device__ __forceinline__ static uint32_t LoadElem(const uint8_t* p)
@@ -102,84 +118,104 @@ struct TileLoader {
         return *p;
     }
 
+    __device__ __forceinline__ static uint4 LoadChunk(const uint8_t* p, bool full, int valid)
+    {
+        if (full) {
+            return *reinterpret_cast<const uint4*>(p);
+        }
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (e < valid) {
+                w[(e * EB) / 4] |= LoadElem(p + e * EB) << (((e * EB) % 4) * 8);
+            }
+        }
+        return make_uint4(w[0], w[1], w[2], w[3]);
+    }
+
     __device__ __forceinline__ void Fetch(const uint8_t* X, int ld, bool kcontig, bool vec, int r0, int rlim,
                                           int k0, int klim)
     {
         const int t = threadIdx.x;
+        if (kcontig) {
 #pragma unroll
-        for (int q = 0; q < kPerThread; ++q) {
-            const int id = t + kThreads * q;
-            uint32_t w[4] = {0u, 0u, 0u, 0u};
-            if (kcontig) {
-                // chunk = 16 bytes along k of one row
+            for (int q = 0; q < kPerThread; ++q) {
+                const int id = t + kThreads * q;
                 const int r = id / (kRowBytes / 16);
-                const int kc = (id % (kRowBytes / 16)) * (16 / EB);
                 const int gr = r0 + r;
-                const int gk = k0 + kc;
+                const int gk = k0 + (id % (kRowBytes / 16)) * E;
                 const uint8_t* p = X + (static_cast<size_t>(gr) * ld + gk) * EB;
-                if (vec && gr < rlim && gk + 16 / EB <= klim) {
-                    v[q] = *reinterpret_cast<const uint4*>(p);
-                    continue;
-                }
-#pragma unroll
-                for (int e = 0; e < 16 / EB; ++e) {
-                    uint32_t x = 0;
-                    if (gr < rlim && gk + e < klim) {
-                        x = LoadElem(p + e * EB);
-                    }
-                    w[(e * EB) / 4] |= x << (((e * EB) % 4) * 8);
-                }
-            } else {
-                // chunk = 16 bytes along rows at one k
-                constexpr int kRowChunks = ROWS * EB / 16;
-                const int kk = id / kRowChunks;
-                const int rc = (id % kRowChunks) * (16 / EB);
-                const int gr = r0 + rc;
-                const int gk = k0 + kk;
-                const uint8_t* p = X + (static_cast<size_t>(gk) * ld + gr) * EB;
-                if (vec && gk < klim && gr + 16 / EB <= rlim) {
-                    v[q] = *reinterpret_cast<const uint4*>(p);
-                    continue;
-                }
-#pragma unroll
-                for (int e = 0; e < 16 / EB; ++e) {
-                    uint32_t x = 0;
-                    if (gk < klim && gr + e < rlim) {
-                        x = LoadElem(p + e * EB);
-                    }
-                    w[(e * EB) / 4] |= x << (((e * EB) % 4) * 8);
-                }
+                const int valid = gr < rlim ? klim - gk : 0;
+                v[q] = LoadChunk(p, vec && valid >= E, valid);
             }
-            v[q] = make_uint4(w[0], w[1], w[2], w[3]);
+        } else {
+            const int mc = t % kMChunks;
+            const int kg = t / kMChunks;
+            const int gr = r0 + mc * E;
+#pragma unroll
+            for (int q = 0; q < kPerThread; ++q) {
+                const int gk = k0 + kg * kPerThread + q;
+                const uint8_t* p = X + (static_cast<size_t>(gk) * ld + gr) * EB;
+                const int valid = gk < klim ? rlim - gr : 0;
+                v[q] = LoadChunk(p, vec && valid >= E, valid);
+            }
         }
     }
 
     __device__ __forceinline__ void Store(uint8_t* lds, bool kcontig) const
     {
         const int t = threadIdx.x;
+        if (kcontig) {
 #pragma unroll
-        for (int q = 0; q < kPerThread; ++q) {
-            const int id = t + kThreads * q;
-            if (kcontig) {
+            for (int q = 0; q < kPerThread; ++q) {
+                const int id = t + kThreads * q;
                 const int r = id / (kRowBytes / 16);
                 const int b = (id % (kRowBytes / 16)) * 16;
                 *reinterpret_cast<uint4*>(lds + SwizzledOffset(r, b)) = v[q];
-            } else {
-                constexpr int kRowChunks = ROWS * EB / 16;
-                const int kk = id / kRowChunks;
-                const int rc = (id % kRowChunks) * (16 / EB);
+            }
+            return;
+        }
+        const int mc = t % kMChunks;
+        const int kg = t / kMChunks;
+        const int byte0 = kg * kPerThread * EB;
+        if constexpr (EB == 4 || EB == 2) {
+            constexpr int kWords = kPerThread * EB / 4;  // 32-bit words per row segment
+            const uint32_t* w = reinterpret_cast<const uint32_t*>(v);
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                uint32_t seg[kWords];
+#pragma unroll
+                for (int o = 0; o < kWords; ++o) {
+                    if constexpr (EB == 4) {
+                        seg[o] = w[o * 4 + i];
+                    } else {
+                        // element i of chunks 2o and 2o+1 (16-bit halves)
+                        const uint32_t lo = w[(2 * o) * 4 + i / 2];
+                        const uint32_t hi = w[(2 * o + 1) * 4 + i / 2];
+                        seg[o] = __builtin_amdgcn_perm(hi, lo, (i & 1) ? 0x07060302u : 0x05040100u);
+                    }
+                }
+                const int row = mc * E + i;
+                if constexpr (kWords >= 4) {
+#pragma unroll
+                    for (int o = 0; o < kWords; o += 4) {
+                        *reinterpret_cast<uint4*>(lds + SwizzledOffset(row, byte0 + o * 4)) =
+                            make_uint4(seg[o], seg[o + 1], seg[o + 2], seg[o + 3]);
+                    }
+                } else if constexpr (kWords == 2) {
+                    *reinterpret_cast<uint2*>(lds + SwizzledOffset(row, byte0)) = make_uint2(seg[0], seg[1]);
+                } else {
+                    *reinterpret_cast<uint32_t*>(lds + SwizzledOffset(row, byte0)) = seg[0];
+                }
+            }
+        } else {
+            // FP8: element stores
+#pragma unroll
+            for (int q = 0; q < kPerThread; ++q) {
                 const uint32_t w[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
 #pragma unroll
-                for (int e = 0; e < 16 / EB; ++e) {
-                    const uint32_t x = (w[(e * EB) / 4] >> (((e * EB) % 4) * 8));
-                    uint8_t* dst = lds + SwizzledOffset(rc + e, kk * EB);
-                    if (EB == 4) {
-                        *reinterpret_cast<uint32_t*>(dst) = x;
-                    } else if (EB == 2) {
-                        *reinterpret_cast<uint16_t*>(dst) = static_cast<uint16_t>(x);
-                    } else {
-                        *dst = static_cast<uint8_t>(x);
-                    }
+                for (int e = 0; e < E; ++e) {
+                    lds[SwizzledOffset(mc * E + e, byte0 + q)] = static_cast<uint8_t>(w[e / 4] >> ((e % 4) * 8));
                 }
             }
         }
@@ -227,30 +263,57 @@ struct Mfma<kF16, kF16> {
     }
 };
 
-// FP8/BF8 operand format codes of v_mfma_f32_16x16x128_f8f6f4 (cbsz / blgp):
-// 0 = E4M3 (fp8), 1 = E5M2 (bf8).
+// FP8 operands with FP32-grade accumulation.  v_mfma_f32_16x16x128_f8f6f4
+// sums its 128 products in a narrow internal window (about 2^-14 of the
+// largest product is kept), which misses the FP32-accumulate bar of GemmEx.
+// Instead every FP8 value is widened to BF16 in registers (exact: E4M3 and
+// E5M2 are subsets of BF16) and fed to v_mfma_f32_16x16x32_bf16, whose products
+// are exact and whose sums are FP32.  The LDS tiles stay 1 byte per element.
+template <int K>
+__device__ __forceinline__ uint32_t Fp8PairToBf16(uint32_t w, bool hi)
+{
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    bf16x2 r;
+    if (K == kE5M2) {
+        r = hi ? __builtin_amdgcn_cvt_scalef32_pk_bf16_bf8(w, 1.0f, true)
+               : __builtin_amdgcn_cvt_scalef32_pk_bf16_bf8(w, 1.0f, false);
+    } else {
+        r = hi ? __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(w, 1.0f, true)
+               : __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(w, 1.0f, false);
+    }
+    return __builtin_bit_cast(uint32_t, r);
+}
+
+// 8 FP8 values (two 32-bit words) -> 8 BF16 values.
+template <int K>
+__device__ __forceinline__ bf16x8 Fp8x8ToBf16(uint32_t w0, uint32_t w1)
+{
+    const uint4 r = make_uint4(Fp8PairToBf16<K>(w0, false), Fp8PairToBf16<K>(w0, true),
+                               Fp8PairToBf16<K>(w1, false), Fp8PairToBf16<K>(w1, true));
+    return __builtin_bit_cast(bf16x8, r);
+}
+
 template <int KA, int KB>
-struct Fp8Mfma {
-    static constexpr int fa = KA == kE5M2 ? 1 : 0;
-    static constexpr int fb = KB == kE5M2 ? 1 : 0;
+struct Fp8ViaBf16Mfma {
     __device__ __forceinline__ static void Run(const uint4 (&a)[2], const uint4 (&b)[2], f32x4& acc)
     {
-        struct P {
-            uint4 x, y;
-        };
-        const i32x8 va = __builtin_bit_cast(i32x8, P{a[0], a[1]});
-        const i32x8 vb = __builtin_bit_cast(i32x8, P{b[0], b[1]});
-        acc = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(va, vb, acc, fa, fb, 0, 0, 0, 0);
+        const uint32_t* wa = reinterpret_cast<const uint32_t*>(a);
+        const uint32_t* wb = reinterpret_cast<const uint32_t*>(b);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Fp8x8ToBf16<KA>(wa[2 * s], wa[2 * s + 1]),
+                                                          Fp8x8ToBf16<KB>(wb[2 * s], wb[2 * s + 1]), acc, 0, 0, 0);
+        }
     }
 };
 template <>
-struct Mfma<kE4M3, kE4M3> : Fp8Mfma<kE4M3, kE4M3> {};
+struct Mfma<kE4M3, kE4M3> : Fp8ViaBf16Mfma<kE4M3, kE4M3> {};
 template <>
-struct Mfma<kE4M3, kE5M2> : Fp8Mfma<kE4M3, kE5M2> {};
+struct Mfma<kE4M3, kE5M2> : Fp8ViaBf16Mfma<kE4M3, kE5M2> {};
 template <>
-struct Mfma<kE5M2, kE4M3> : Fp8Mfma<kE5M2, kE4M3> {};
+struct Mfma<kE5M2, kE4M3> : Fp8ViaBf16Mfma<kE5M2, kE4M3> {};
 template <>
-struct Mfma<kE5M2, kE5M2> : Fp8Mfma<kE5M2, kE5M2> {};
+struct Mfma<kE5M2, kE5M2> : Fp8ViaBf16Mfma<kE5M2, kE5M2> {};
 
 // Output element conversions (round to nearest even).
 __device__ __forceinline__ uint16_t ToBf16(float f)
@@ -292,17 +355,56 @@ __device__ __forceinline__ void StoreC(void* C, size_t idx, float v)
     }
 }
 
-// Block-id to tile mapping.  Workgroups are dispatched round-robin over the 8
-// XCDs (each with its own L2); consecutive ids of one XCD are given
-// neighbouring tiles, grouped by GROUP rows of tiles so that a group shares its
-// A row-panels and B column-panels in that XCD's L2.
-__device__ __forceinline__ void TileOf(int bid, int nblocks, int tilesM, int tilesN, int& tm, int& tn)
+// Four consecutive elements of one column of C (rows row..row+3); `full`
+// selects one vector access, otherwise the first `valid` elements move singly.
+template <int OK>
+__device__ __forceinline__ void LoadC4(const void* C, size_t idx, bool full, int valid, float (&c)[4])
 {
-    constexpr int kXcd = 8;
-    int id = bid;
-    if (nblocks % kXcd == 0) {
-        id = (bid % kXcd) * (nblocks / kXcd) + bid / kXcd;
+    if (full) {
+        if (OK == kOutF32) {
+            const float4 x = *reinterpret_cast<const float4*>(static_cast<const float*>(C) + idx);
+            c[0] = x.x; c[1] = x.y; c[2] = x.z; c[3] = x.w;
+        } else {
+            const uint2 x = *reinterpret_cast<const uint2*>(static_cast<const uint16_t*>(C) + idx);
+            const uint16_t h[4] = {static_cast<uint16_t>(x.x), static_cast<uint16_t>(x.x >> 16),
+                                   static_cast<uint16_t>(x.y), static_cast<uint16_t>(x.y >> 16)};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) c[e] = OK == kOutBF16 ? FromBf16(h[e]) : FromF16(h[e]);
+        }
+        return;
     }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) c[e] = e < valid ? LoadC<OK>(C, idx + e) : 0.0f;
+}
+
+template <int OK>
+__device__ __forceinline__ void StoreC4(void* C, size_t idx, bool full, int valid, const float (&v)[4])
+{
+    if (full) {
+        if (OK == kOutF32) {
+            *reinterpret_cast<float4*>(static_cast<float*>(C) + idx) = make_float4(v[0], v[1], v[2], v[3]);
+        } else {
+            uint16_t h[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) h[e] = OK == kOutBF16 ? ToBf16(v[e]) : ToF16(v[e]);
+            *reinterpret_cast<uint2*>(static_cast<uint16_t*>(C) + idx) =
+                make_uint2(h[0] | (static_cast<uint32_t>(h[1]) << 16), h[2] | (static_cast<uint32_t>(h[3]) << 16));
+        }
+        return;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        if (e < valid) StoreC<OK>(C, idx + e, v[e]);
+    }
+}
+
+// Tile of one problem from its (XCD-ordered) block index.  Workgroups are
+// dispatched round-robin over the 8 XCDs (each with its own L2); the kernel
+// first renumbers blocks so that consecutive numbers run on one XCD, then
+// walks tiles in groups of GROUP tile-rows so that the blocks of one XCD share
+// their A row-panels and B column-panels in that XCD's L2.
+__device__ __forceinline__ void TileOf(int id, int tilesM, int tilesN, int& tm, int& tn)
+{
     constexpr int kGroup = 8;
     const int perGroup = kGroup * tilesN;
     const int g = id / perGroup;
@@ -331,8 +433,15 @@ __global__ __launch_bounds__(kThreads) void GemmKernel(GemmArgs p)
     __shared__ __attribute__((aligned(16))) uint8_t sA[2][BM * kRowBytes];
     __shared__ __attribute__((aligned(16))) uint8_t sB[2][BN * kRowBytes];
 
+    // XCD-aware order over the whole grid, then (problem, tile) for batches
+    int bid = blockIdx.x;
+    if (gridDim.x % 8 == 0) {
+        bid = (bid % 8) * (gridDim.x / 8) + bid / 8;
+    }
+    const int tilesPer = p.tilesM * p.tilesN;
+    const int prob = bid / tilesPer;
     int tm, tn;
-    TileOf(blockIdx.x, gridDim.x, p.tilesM, p.tilesN, tm, tn);
+    TileOf(bid - prob * tilesPer, p.tilesM, p.tilesN, tm, tn);
     const int m0 = tm * BM;
     const int n0 = tn * BN;
     const int t = threadIdx.x;
@@ -341,8 +450,13 @@ __global__ __launch_bounds__(kThreads) void GemmKernel(GemmArgs p)
     const int wm = (wave & 1) * WM;
     const int wn = (wave >> 1) * WN;
 
-    const uint8_t* A = static_cast<const uint8_t*>(p.A);
-    const uint8_t* B = static_cast<const uint8_t*>(p.B);
+    const uint8_t* A = static_cast<const uint8_t*>(p.Aarr != nullptr ? p.Aarr[prob] : p.A);
+    const uint8_t* B = static_cast<const uint8_t*>(p.Barr != nullptr ? p.Barr[prob] : p.B);
+    void* C = p.Carr != nullptr ? p.Carr[prob] : p.C;
+    // per-problem pointers of a batch may be less aligned than the host assumed
+    const bool vecA = p.vecA && (reinterpret_cast<uintptr_t>(A) % 16 == 0);
+    const bool vecB = p.vecB && (reinterpret_cast<uintptr_t>(B) % 16 == 0);
+    const bool vecC = p.vecC && (reinterpret_cast<uintptr_t>(C) % 16 == 0);
     // op(A) row r = C row; k-contiguous when A is transposed.
     const bool aK = p.transA != 0;
     // op(B) column c = C column; k-contiguous when B is not transposed.
@@ -359,8 +473,8 @@ __global__ __launch_bounds__(kThreads) void GemmKernel(GemmArgs p)
 
     const int nk = (p.k + kK - 1) / kK;
     if (nk > 0) {
-        la.Fetch(A, p.lda, aK, p.vecA, m0, p.m, 0, p.k);
-        lb.Fetch(B, p.ldb, bK, p.vecB, n0, p.n, 0, p.k);
+        la.Fetch(A, p.lda, aK, vecA, m0, p.m, 0, p.k);
+        lb.Fetch(B, p.ldb, bK, vecB, n0, p.n, 0, p.k);
         la.Store(sA[0], aK);
         lb.Store(sB[0], bK);
     }
@@ -373,8 +487,8 @@ __global__ __launch_bounds__(kThreads) void GemmKernel(GemmArgs p)
         const int cur = kb & 1;
         const bool more = kb + 1 < nk;
         if (more) {
-            la.Fetch(A, p.lda, aK, p.vecA, m0, p.m, (kb + 1) * kK, p.k);
-            lb.Fetch(B, p.ldb, bK, p.vecB, n0, p.n, (kb + 1) * kK, p.k);
+            la.Fetch(A, p.lda, aK, vecA, m0, p.m, (kb + 1) * kK, p.k);
+            lb.Fetch(B, p.ldb, bK, vecB, n0, p.n, (kb + 1) * kK, p.k);
         }
         uint4 fb[TN][2];
 #pragma unroll
@@ -413,23 +527,33 @@ __global__ __launch_bounds__(kThreads) void GemmKernel(GemmArgs p)
         for (int j = 0; j < TN; ++j) {
             const int col = n0 + wn + j * 16 + fr;
             const int row = m0 + wm + i * 16 + rq;
-            if (col >= p.n) {
+            if (col >= p.n || row >= p.m) {
                 continue;
             }
             const size_t base = static_cast<size_t>(col) * p.ldc + row;
+            float v[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                if (row + e < p.m) {
-                    float v = alpha * acc[i][j][e];
-                    if (beta != 0.0f) {
-                        v = v + beta * LoadC<OK>(p.C, base + e);
-                    }
-                    if (kFp8 && !__builtin_isfinite(v)) {
-                        v = __builtin_nanf("");
-                    }
-                    StoreC<OK>(p.C, base + e, v);
+                v[e] = alpha * acc[i][j][e];
+            }
+            const bool full = vecC && row + 4 <= p.m;
+            if (beta != 0.0f) {
+                float c[4];
+                LoadC4<OK>(C, base, full, p.m - row, c);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    v[e] = v[e] + beta * c[e];
                 }
             }
+            if (kFp8) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    if (!__builtin_isfinite(v[e])) {
+                        v[e] = __builtin_nanf("");
+                    }
+                }
+            }
+            StoreC4<OK>(C, base, full, p.m - row, v);
         }
     }
 }

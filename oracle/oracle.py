@@ -268,4 +268,8 @@ def mere_mare(out, gold, threshold, multiplier=10.0):
     mere = float(rel.mean()) if rel.size else 0.0
     mare = float(rel.max()) if rel.size else 0.0
     ok = mism == 0 and mere < threshold and mare < multiplier * threshold
-    return ok, "MERE=%.3e MARE=%.3e mismatches=%d" % (mere, mare, mism)
+    detail = "MERE=%.3e MARE=%.3e mismatches=%d differing=%d" % (mere, mare, mism, int(rel.size))
+    if rel.size:
+        w = int(np.argmax(rel))
+        detail += " worst out=%.9g gold=%.9g" % (o[w], g[w])
+    return ok, detail

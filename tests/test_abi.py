@@ -256,3 +256,39 @@ def test_gemm_ex_extensions_are_accepted():
                    B.ACL_FLOAT16, 1, B.COMPUTE["COMPUTE_32F"])
     assert st == status_code("NOT_SUPPORTED")
     h.destroy()
+
+
+BATCHED_ROWS = [r for r in load_cases("gemm_batched_ex.csv")
+                if status_code(r["expect_result"]) != 0 or _gemm_degenerate(r) or int(r["batchCount"]) == 0]
+
+
+@pytest.mark.parametrize("row", BATCHED_ROWS, ids=[r["case_name"] for r in BATCHED_ROWS])
+def test_gemm_batched_ex_argument_rows(cpu_handle, row):
+    exp = status_code(row["expect_result"])
+    h = None if exp == status_code("HANDLE_IS_NULLPTR") else cpu_handle
+    alpha = None if _bool(row["alpha_null"]) else float(row["alpha"])
+    beta = None if _bool(row["beta_null"]) else float(row["beta"])
+    A = None if _bool(row["aarray_null"]) else DUMMY
+    Bp = None if _bool(row["barray_null"]) else DUMMY
+    C = None if _bool(row["carray_null"]) else DUMMY
+    lda, ldb, ldc = gemm_ex_lds(row)
+    algo = 0 if row["algo"] == "DEFAULT" else 1 + int(row["algo"].replace("ALGO", ""))
+    st = B.gemm_batched_ex(h, _op(row["transA"]), _op(row["transB"]), int(row["m"]), int(row["n"]), int(row["k"]),
+                           alpha, A, B.DTYPES[row["Atype"]], lda, Bp, B.DTYPES[row["Btype"]], ldb, beta, C,
+                           B.DTYPES[row["Ctype"]], ldc, int(row["batchCount"]), B.COMPUTE[row["computeType"]], algo)
+    assert st == exp
+
+
+def test_sgemm_batched_argument_checks(cpu_handle):
+    ok = status_code("SUCCESS")
+    inv = status_code("INVALID_VALUE")
+    f = B.sgemm_batched
+    assert f(None, B.OP_N, B.OP_N, 4, 4, 4, 1.0, DUMMY, 4, DUMMY, 4, 0.0, DUMMY, 4, 1) == status_code("HANDLE_IS_NULLPTR")
+    assert f(cpu_handle, B.OP_N, B.OP_N, 4, 4, 4, None, DUMMY, 4, DUMMY, 4, 0.0, DUMMY, 4, 1) == inv
+    assert f(cpu_handle, B.OP_N, B.OP_N, -1, 4, 4, 1.0, DUMMY, 4, DUMMY, 4, 0.0, DUMMY, 4, 1) == inv
+    assert f(cpu_handle, B.OP_N, B.OP_N, 4, 4, 4, 1.0, DUMMY, 4, DUMMY, 4, 0.0, DUMMY, 4, -1) == inv
+    assert f(cpu_handle, 999, B.OP_N, 4, 4, 4, 1.0, DUMMY, 4, DUMMY, 4, 0.0, DUMMY, 4, 1) == inv
+    assert f(cpu_handle, B.OP_N, B.OP_N, 4, 4, 4, 1.0, DUMMY, 3, DUMMY, 4, 0.0, DUMMY, 4, 1) == inv
+    assert f(cpu_handle, B.OP_N, B.OP_N, 4, 4, 4, 1.0, None, 4, DUMMY, 4, 0.0, DUMMY, 4, 1) == inv
+    assert f(cpu_handle, B.OP_N, B.OP_N, 0, 4, 4, 1.0, DUMMY, 4, DUMMY, 4, 0.0, DUMMY, 4, 1) == ok
+    assert f(cpu_handle, B.OP_N, B.OP_N, 4, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 4, 0) == ok

@@ -93,9 +93,20 @@ def test_getrf_c2_config(handle):
     assert st == 0 and inf.value == 0
     X = to_host(dX)
     assert np.array_equal(X, O.getrs_batched(0, gLU, gpiv, rhs))
-    exact = np.linalg.solve(A.transpose(0, 2, 1).astype(np.float64), rhs.transpose(0, 2, 1).astype(np.float64))
+    # FP64 oracle: normwise backward error of every solve at FP32 level, and
+    # forward error within 1e-5 wherever the system allows it (cond_inf <= 1e3,
+    # about 3/4 of these systems); elsewhere bounded by cond * u (FP32 LU is
+    # backward stable, not exact).
+    berr = O.backward_error(A, rhs, X)
+    assert berr.max() <= 1e-6, berr.max()
+    M = A.transpose(0, 2, 1).astype(np.float64)
+    exact = np.linalg.solve(M, rhs.transpose(0, 2, 1).astype(np.float64))
     rel = np.abs(X.transpose(0, 2, 1) - exact).max(axis=(1, 2)) / np.abs(exact).max(axis=(1, 2))
-    assert rel.max() <= 1e-5, rel.max()
+    cond = np.linalg.cond(M, p=np.inf)
+    well = cond <= 1e3
+    assert well.sum() > batch // 2
+    assert rel[well].max() <= 1e-5, rel[well].max()
+    assert np.all(rel <= cond * 2.0 ** -24), (rel / cond).max()
     del torch
 
 
