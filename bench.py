@@ -34,6 +34,21 @@ GETRF_BYTES = 2 * N * N * 4 + N * 4 + 4 + 8
 GETRS_BYTES = N * N * 4 + N * 4 + 2 * N * NRHS * 4 + 16
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "getrf_wave32_batch4096_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per getrf launch from the committed FETCH_SIZE/WRITE_SIZE summary of this workload."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,7 +157,7 @@ def main():
         "config": {"workload": "FP32 aclblasSgetrfBatched + aclblasSgetrsBatched, batch 4096 x 32x32, nrhs 1",
                    "batch": BATCH, "n": N, "nrhs": NRHS, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "getrf (n<=32 wave kernel)", "kernel_ms": getrf_ms,
                      "algorithmic_bytes_per_launch": GETRF_BYTES * BATCH},
         "max_backward_error": berr,
