@@ -52,6 +52,50 @@ __device__ __forceinline__ uint64_t WaveMaxKey(uint64_t k)
     return k;
 }
 
+// DPP lane exchange inside 16-lane rows (no LDS traffic).  Controls used here:
+// quad_perm [1,0,3,2] = 0xB1 (lane ^ 1), quad_perm [2,3,0,1] = 0x4E (lane ^ 2),
+// row_half_mirror = 0x141 (lane i <-> 7 - i within each 8 lanes), row_mirror =
+// 0x140 (lane i <-> 15 - i within each 16 lanes).  Every source lane exists.
+template <int CTRL>
+__device__ __forceinline__ uint32_t Dpp(uint32_t v)
+{
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, 0xF, 0xF, false));
+}
+
+// Maximum (MAX = true) or minimum of v over aligned groups of S lanes
+// (S = 8, 16, 32 or 64); every lane of a group receives its group's result.
+template <int S, bool MAX>
+__device__ __forceinline__ uint32_t SegReduce(uint32_t v)
+{
+    static_assert(S == 8 || S == 16 || S == 32 || S == 64, "segment of 8, 16, 32 or 64 lanes");
+    auto op = [](uint32_t x, uint32_t y) { return MAX ? (x > y ? x : y) : (x < y ? x : y); };
+    v = op(v, Dpp<0xB1>(v));
+    v = op(v, Dpp<0x4E>(v));
+    v = op(v, Dpp<0x141>(v));
+    if constexpr (S >= 16) {
+        v = op(v, Dpp<0x140>(v));
+    }
+    if constexpr (S >= 32) {
+        const uint32_t r0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 0));
+        const uint32_t r1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 16));
+        const uint32_t r2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 32));
+        const uint32_t r3 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 48));
+        if constexpr (S == 32) {
+            v = (threadIdx.x & 32) ? op(r2, r3) : op(r0, r1);
+        } else {
+            v = op(op(r0, r1), op(r2, r3));
+        }
+    }
+    return v;
+}
+
+// |v| as an order-preserving integer key; NaN maps to 0 so that it never wins.
+__device__ __forceinline__ uint32_t MagKey(float v)
+{
+    const uint32_t m = __float_as_uint(v) & 0x7FFFFFFFu;
+    return m > 0x7F800000u ? 0u : m;
+}
+
 // Compile-time loop: f(std::integral_constant<int, I>) for I in [Begin, End).
 // Register arrays indexed by I stay in registers whatever the loop size.
 template <int Begin, int End, typename F>

@@ -19,6 +19,7 @@
 // result does not depend on the blocking.
 #include "device_common.h"
 #include "internal.h"
+#include "lapack_kernels.h"
 
 namespace aclblas {
 namespace {
@@ -424,6 +425,9 @@ aclblasStatus_t ValidateGetri(int n, int lda, int ldc, int batch, const float* c
 aclblasStatus_t GetrsBatchedLaunch(hipStream_t st, bool trans, int n, int nrhs, const float* const* A, int lda,
                                    const int* piv, float* const* B, int ldb, int batch)
 {
+    if (LaunchGetrsSmall(st, trans, n, nrhs, A, lda, piv, B, ldb, batch)) {
+        return LaunchStatus("aclblasSgetrsBatched");
+    }
     const int rc = ChooseRc(n, nrhs);
     if (rc <= 0) {
         return ACLBLAS_STATUS_NOT_SUPPORTED;

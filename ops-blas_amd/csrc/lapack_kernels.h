@@ -14,4 +14,8 @@ void LaunchGetrfWave(hipStream_t st, float* const* A, int n, int lda, int* ipiv,
 template <bool PIVOT>
 void LaunchGetrfBlock(hipStream_t st, float* const* A, int n, int lda, int* ipiv, int* info, int batch);
 
+// Triangular solves, n <= 64 (getrs_small.hip); false when the shape is not covered.
+bool LaunchGetrsSmall(hipStream_t st, bool trans, int n, int nrhs, const float* const* A, int lda, const int* piv,
+                      float* const* B, int ldb, int batch);
+
 }  // namespace aclblas
