@@ -115,6 +115,21 @@ aclblasStatus_t aclblasSgemmBatched(
     const float* beta, float* const Carray[], int ldc,
     int batchCount);
 
+/* reference include/cann_ops_blas.h:448-455.  FP32 GEMMs grouped by shape:
+ * group g holds groupSizeArray[g] problems of shape (mArray[g], nArray[g],
+ * kArray[g]) with transaArray[g]/transbArray[g] (N or T; C -> INVALID_ENUM),
+ * alphaArray[g]/betaArray[g] (host floats) and ldaArray[g].. (host arrays).
+ * Aarray/Barray/Carray are HOST arrays of device pointers, one per problem,
+ * in group order.  Groups with groupSize, m or n equal to 0 are skipped;
+ * alpha == 0 scales C by beta.  One persistent launch; the problem table is
+ * copied to the handle's workspace asynchronously. */
+aclblasStatus_t aclblasSgemmGroupedBatched(
+    aclblasHandle_t handle, int groupCount, const aclblasOperation_t* transaArray,
+    const aclblasOperation_t* transbArray, const int* mArray, const int* nArray, const int* kArray,
+    const float* alphaArray, const float* const* Aarray, const int* ldaArray, const float* const* Barray,
+    const int* ldbArray, const float* betaArray, float* const* Carray, const int* ldcArray,
+    const int* groupSizeArray);
+
 /* ------------------------------------------------------------------------ */
 /* Batched LAPACK (FP32)                                                     */
 /* ------------------------------------------------------------------------ */
@@ -141,6 +156,25 @@ aclblasStatus_t aclblasSgetrsBatched(
 aclblasStatus_t aclblasSgetriBatched(
     aclblasHandle_t handle, int n, const float* const Aarray[], int lda, const int* PivotArray,
     float* const Carray[], int ldc, int* infoArray, int batchSize);
+
+/* ------------------------------------------------------------------------ */
+/* Differentiation                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* Reverse-mode derivative of X = A^-1 B (no counterpart in the reference; it
+ * is built from the reference's transposed getrs,
+ * blas/getrs_batched/arch35/sgetrs_batched_kernel.cpp:263-362, and an FP32
+ * batched GEMM, blas/gemm_batched/arch35/gemm_batched_host.cpp:278-360).
+ * LUarray/devIpiv: the factors and pivots from aclblasSgetrfBatched (devIpiv
+ * NULL for the unpivoted factorisation).  Xarray: the forward solution (n x
+ * nrhs).  dXarray: on entry the upstream gradient dL/dX, on exit dL/dB =
+ * A^-T dX.  dAarray: on exit dL/dA = -dL/dB X^T (n x n, overwritten); NULL
+ * skips it.  All arrays are device arrays of device pointers.  Status rules
+ * as for aclblasSgetrsBatched, with every bad argument INVALID_VALUE. */
+aclblasStatus_t aclblasSgetrsBatchedAdjoint(
+    aclblasHandle_t handle, int n, int nrhs, const float* const LUarray[], int lda, const int* devIpiv,
+    const float* const Xarray[], int ldx, float* const dXarray[], int lddx, float* const dAarray[], int ldda,
+    int batchCount);
 
 #ifdef __cplusplus
 }

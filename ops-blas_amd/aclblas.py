@@ -60,6 +60,8 @@ _SIGNATURES = {
     "aclblasSgetrfBatched": [_P, _I, _P, _I, _P, _P, _I],
     "aclblasSgetrsBatched": [_P, _I, _I, _I, _P, _I, _P, _P, _I, _P, _I],
     "aclblasSgetriBatched": [_P, _I, _P, _I, _P, _P, _I, _P, _I],
+    "aclblasSgetrsBatchedAdjoint": [_P, _I, _I, _P, _I, _P, _P, _I, _P, _I, _P, _I, _I],
+    "aclblasSgemmGroupedBatched": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
 }
 
 
@@ -158,6 +160,35 @@ def sgetrs_batched(h, trans, n, nrhs, Aarray, lda, piv, Barray, ldb, info_host, 
 
 def sgetri_batched(h, n, Aarray, lda, piv, Carray, ldc, info, batch):
     return load().aclblasSgetriBatched(h, n, Aarray, lda, piv, Carray, ldc, info, batch)
+
+
+def _host_array(ctype, values):
+    if values is None:
+        return None
+    arr = (ctype * max(1, len(values)))(*values)
+    return arr
+
+
+def sgemm_grouped_batched(h, transa, transb, m, n, k, alpha, A, lda, Bp, ldb, beta, C, ldc, group_size):
+    """All arguments are Python lists (host arrays) or None; A/Bp/C are lists of device addresses."""
+    args = [_host_array(ctypes.c_int, transa), _host_array(ctypes.c_int, transb), _host_array(ctypes.c_int, m),
+            _host_array(ctypes.c_int, n), _host_array(ctypes.c_int, k), _host_array(ctypes.c_float, alpha),
+            _host_array(ctypes.c_void_p, A), _host_array(ctypes.c_int, lda), _host_array(ctypes.c_void_p, Bp),
+            _host_array(ctypes.c_int, ldb), _host_array(ctypes.c_float, beta), _host_array(ctypes.c_void_p, C),
+            _host_array(ctypes.c_int, ldc), _host_array(ctypes.c_int, group_size)]
+    count = len(group_size) if group_size is not None else (len(m) if m is not None else 0)
+    return sgemm_grouped_batched_raw(h, count, *args)
+
+
+def sgemm_grouped_batched_raw(h, group_count, *arrays):
+    ptrs = [ctypes.cast(a, ctypes.c_void_p) if a is not None else None for a in arrays]
+    return load().aclblasSgemmGroupedBatched(h, group_count, *ptrs)
+
+
+def sgetrs_batched_adjoint(h, n, nrhs, LUarray, lda, piv, Xarray, ldx, dXarray, lddx, dAarray, ldda, batch):
+    """dX <- A^-T dX (in place, the gradient of B) and dA <- -(A^-T dX) X^T; dAarray may be None."""
+    return load().aclblasSgetrsBatchedAdjoint(h, n, nrhs, LUarray, lda, piv, Xarray, ldx, dXarray, lddx, dAarray,
+                                              ldda, batch)
 
 
 # ---------------------------------------------------------------- torch helpers

@@ -1,0 +1,23 @@
+// Internal launchers shared between the C entry points (no argument checks:
+// callers validate first).  All enqueue on `st` and return the launch status.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cann_ops_blas.h"
+
+namespace aclblas {
+
+// LU with partial pivoting (piv == nullptr: no pivoting), getrf.hip.
+aclblasStatus_t GetrfBatchedLaunch(hipStream_t st, int n, float* const* A, int lda, int* piv, int* info, int batch);
+
+// op(A) X = B with LU factors, B overwritten (trans: A^T), getrs.hip.
+aclblasStatus_t GetrsBatchedLaunch(hipStream_t st, bool trans, int n, int nrhs, const float* const* A, int lda,
+                                   const int* piv, float* const* B, int ldb, int batch);
+
+// C[i] = alpha op(A[i]) op(B[i]) + beta C[i], FP32, device pointer arrays, gemm.hip.
+aclblasStatus_t SgemmBatchedLaunch(hipStream_t st, int cus, bool transA, bool transB, int m, int n, int k,
+                                   float alpha, const float* const* A, int lda, const float* const* B, int ldb,
+                                   float beta, float* const* C, int ldc, int batch);
+
+}  // namespace aclblas

@@ -11,6 +11,7 @@
 // whenever alpha != 1 or beta != 0; here alpha and beta are applied in the
 // MFMA kernel's epilogue, so no workspace is needed and C is read at most once.
 #include "gemm_kernel.h"
+#include "drivers.h"
 #include "internal.h"
 
 namespace aclblas {
@@ -52,12 +53,16 @@ void LaunchScale(hipStream_t st, int outKind, void* C, int m, int n, int ldc, fl
 }
 
 // C_b[:m, :n] = beta * C_b for every problem of a batch (pointer array).
+// Grid: perProblem blocks for each problem, problems along x (no 65535 limit).
 template <int OK>
-__global__ __launch_bounds__(256) void ScaleCBatchedKernel(void* const* Carr, int m, int n, int ldc, float beta)
+__global__ __launch_bounds__(256) void ScaleCBatchedKernel(void* const* Carr, int m, int n, int ldc, float beta,
+                                                            int perProblem)
 {
-    void* C = Carr[blockIdx.y];
+    const int prob = blockIdx.x / perProblem;
+    const int part = blockIdx.x - prob * perProblem;
+    void* C = Carr[prob];
     const long long total = static_cast<long long>(m) * n;
-    for (long long e = blockIdx.x * 256ll + threadIdx.x; e < total; e += 256ll * gridDim.x) {
+    for (long long e = part * 256ll + threadIdx.x; e < total; e += 256ll * perProblem) {
         const int j = static_cast<int>(e / m);
         const int i = static_cast<int>(e - static_cast<long long>(j) * m);
         const size_t idx = static_cast<size_t>(j) * ldc + i;
@@ -71,11 +76,20 @@ void LaunchScaleBatched(hipStream_t st, int outKind, void* const* Carr, int m, i
     const long long total = static_cast<long long>(m) * n;
     long long bx = (total + 255) / 256;
     bx = bx > 64 ? 64 : bx;
-    const dim3 grid(static_cast<unsigned>(bx), static_cast<unsigned>(batch));
+    const long long maxPer = 0x7FFFFFFFll / (batch > 0 ? batch : 1);
+    bx = bx > maxPer ? (maxPer > 0 ? maxPer : 1) : bx;
+    const int per = static_cast<int>(bx);
+    const dim3 grid(static_cast<unsigned>(bx * batch));
     switch (outKind) {
-        case kOutF32: hipLaunchKernelGGL((ScaleCBatchedKernel<kOutF32>), grid, dim3(256), 0, st, Carr, m, n, ldc, beta); break;
-        case kOutBF16: hipLaunchKernelGGL((ScaleCBatchedKernel<kOutBF16>), grid, dim3(256), 0, st, Carr, m, n, ldc, beta); break;
-        default: hipLaunchKernelGGL((ScaleCBatchedKernel<kOutF16>), grid, dim3(256), 0, st, Carr, m, n, ldc, beta); break;
+        case kOutF32:
+            hipLaunchKernelGGL((ScaleCBatchedKernel<kOutF32>), grid, dim3(256), 0, st, Carr, m, n, ldc, beta, per);
+            break;
+        case kOutBF16:
+            hipLaunchKernelGGL((ScaleCBatchedKernel<kOutBF16>), grid, dim3(256), 0, st, Carr, m, n, ldc, beta, per);
+            break;
+        default:
+            hipLaunchKernelGGL((ScaleCBatchedKernel<kOutF16>), grid, dim3(256), 0, st, Carr, m, n, ldc, beta, per);
+            break;
     }
 }
 
@@ -481,6 +495,23 @@ aclblasStatus_t RunBatched(aclblasHandle_t handle, aclDataType at, aclDataType b
 }
 
 }  // namespace
+
+aclblasStatus_t SgemmBatchedLaunch(hipStream_t st, int cus, bool transA, bool transB, int m, int n, int k,
+                                   float alpha, const float* const* A, int lda, const float* const* B, int ldb,
+                                   float beta, float* const* C, int ldc, int batch)
+{
+    if (k == 0 || alpha == 0.0f) {
+        if (beta == 1.0f) {
+            return ACLBLAS_STATUS_SUCCESS;
+        }
+        LaunchScaleBatched(st, kOutF32, reinterpret_cast<void* const*>(C), m, n, ldc, beta, batch);
+        return LaunchStatus("aclblasSgemmBatched");
+    }
+    return RunGemm(st, cus, kF32, kF32, kOutF32, transA, transB, m, n, k, alpha, nullptr, lda, nullptr, ldb, beta,
+                   nullptr, ldc, nullptr, nullptr, reinterpret_cast<const void* const*>(A),
+                   reinterpret_cast<const void* const*>(B), reinterpret_cast<void* const*>(C), batch);
+}
+
 }  // namespace aclblas
 
 extern "C" aclblasStatus_t aclblasGemmBatchedEx(

@@ -415,10 +415,25 @@ __device__ __forceinline__ void TileOf(int id, int tilesM, int tilesN, int& tm, 
     tn = inGroup / rowsInGroup;
 }
 
-// FP8 non-finite results become NaN (reference test/gemm/gemm_ex/gemm_ex_golden.h,
-// QuantizeGoldenOutput: the reference's FP8 path yields NaN for non-finite).
+// One BM x BN tile of one problem: everything the tile computation needs.
+struct TileJob {
+    const uint8_t* A;
+    const uint8_t* B;
+    void* C;
+    int m, n, k;
+    int lda, ldb, ldc;
+    bool aK, bK;            // operand tiles are k-contiguous in memory
+    bool vecA, vecB, vecC;  // 16-byte operand loads / 4-element C moves are legal
+    float alpha, beta;
+};
+
+// Computes C[m0:m0+BM, n0:n0+BN] = alpha * op(A) op(B) + beta * C for one job
+// with the LDS stages sA/sB.  FP8 non-finite results become NaN (reference
+// test/gemm/gemm_ex/gemm_ex_golden.h, QuantizeGoldenOutput: the reference's FP8
+// path yields NaN for non-finite).  Ends with the LDS free for the next tile.
 template <int KA, int KB, int OK, int BM, int BN>
-__global__ __launch_bounds__(kThreads) void GemmKernel(GemmArgs p)
+__device__ __forceinline__ void GemmTile(const TileJob& q, int m0, int n0, uint8_t (*sA)[BM * kRowBytes],
+                                         uint8_t (*sB)[BN * kRowBytes])
 {
     constexpr int EA = KindInfo<KA>::bytes;
     constexpr int EB = KindInfo<KB>::bytes;
@@ -430,37 +445,11 @@ __global__ __launch_bounds__(kThreads) void GemmKernel(GemmArgs p)
     constexpr int TN = WN / 16;
     constexpr bool kFp8 = (KA == kE4M3 || KA == kE5M2);
 
-    __shared__ __attribute__((aligned(16))) uint8_t sA[2][BM * kRowBytes];
-    __shared__ __attribute__((aligned(16))) uint8_t sB[2][BN * kRowBytes];
-
-    // XCD-aware order over the whole grid, then (problem, tile) for batches
-    int bid = blockIdx.x;
-    if (gridDim.x % 8 == 0) {
-        bid = (bid % 8) * (gridDim.x / 8) + bid / 8;
-    }
-    const int tilesPer = p.tilesM * p.tilesN;
-    const int prob = bid / tilesPer;
-    int tm, tn;
-    TileOf(bid - prob * tilesPer, p.tilesM, p.tilesN, tm, tn);
-    const int m0 = tm * BM;
-    const int n0 = tn * BN;
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wave = t >> 6;
     const int wm = (wave & 1) * WM;
     const int wn = (wave >> 1) * WN;
-
-    const uint8_t* A = static_cast<const uint8_t*>(p.Aarr != nullptr ? p.Aarr[prob] : p.A);
-    const uint8_t* B = static_cast<const uint8_t*>(p.Barr != nullptr ? p.Barr[prob] : p.B);
-    void* C = p.Carr != nullptr ? p.Carr[prob] : p.C;
-    // per-problem pointers of a batch may be less aligned than the host assumed
-    const bool vecA = p.vecA && (reinterpret_cast<uintptr_t>(A) % 16 == 0);
-    const bool vecB = p.vecB && (reinterpret_cast<uintptr_t>(B) % 16 == 0);
-    const bool vecC = p.vecC && (reinterpret_cast<uintptr_t>(C) % 16 == 0);
-    // op(A) row r = C row; k-contiguous when A is transposed.
-    const bool aK = p.transA != 0;
-    // op(B) column c = C column; k-contiguous when B is not transposed.
-    const bool bK = p.transB == 0;
 
     TileLoader<BM, EA> la;
     TileLoader<BN, EB> lb;
@@ -471,12 +460,12 @@ __global__ __launch_bounds__(kThreads) void GemmKernel(GemmArgs p)
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    const int nk = (p.k + kK - 1) / kK;
+    const int nk = (q.k + kK - 1) / kK;
     if (nk > 0) {
-        la.Fetch(A, p.lda, aK, vecA, m0, p.m, 0, p.k);
-        lb.Fetch(B, p.ldb, bK, vecB, n0, p.n, 0, p.k);
-        la.Store(sA[0], aK);
-        lb.Store(sB[0], bK);
+        la.Fetch(q.A, q.lda, q.aK, q.vecA, m0, q.m, 0, q.k);
+        lb.Fetch(q.B, q.ldb, q.bK, q.vecB, n0, q.n, 0, q.k);
+        la.Store(sA[0], q.aK);
+        lb.Store(sB[0], q.bK);
     }
     __syncthreads();
 
@@ -487,8 +476,8 @@ __global__ __launch_bounds__(kThreads) void GemmKernel(GemmArgs p)
         const int cur = kb & 1;
         const bool more = kb + 1 < nk;
         if (more) {
-            la.Fetch(A, p.lda, aK, vecA, m0, p.m, (kb + 1) * kK, p.k);
-            lb.Fetch(B, p.ldb, bK, vecB, n0, p.n, (kb + 1) * kK, p.k);
+            la.Fetch(q.A, q.lda, q.aK, q.vecA, m0, q.m, (kb + 1) * kK, q.k);
+            lb.Fetch(q.B, q.ldb, q.bK, q.vecB, n0, q.n, (kb + 1) * kK, q.k);
         }
         uint4 fb[TN][2];
 #pragma unroll
@@ -509,17 +498,15 @@ __global__ __launch_bounds__(kThreads) void GemmKernel(GemmArgs p)
             }
         }
         if (more) {
-            la.Store(sA[cur ^ 1], aK);
-            lb.Store(sB[cur ^ 1], bK);
+            la.Store(sA[cur ^ 1], q.aK);
+            lb.Store(sB[cur ^ 1], q.bK);
         }
         __syncthreads();
     }
 
-    // ---- fused epilogue: C = alpha * scale * acc + beta * C ----
-    float alpha = p.alpha;
-    if (p.scaleA != nullptr) alpha *= *p.scaleA;
-    if (p.scaleB != nullptr) alpha *= *p.scaleB;
-    const float beta = p.beta;
+    // ---- fused epilogue: C = alpha * acc + beta * C ----
+    const float alpha = q.alpha;
+    const float beta = q.beta;
     const int rq = (lane >> 4) * 4;
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
@@ -527,19 +514,19 @@ __global__ __launch_bounds__(kThreads) void GemmKernel(GemmArgs p)
         for (int j = 0; j < TN; ++j) {
             const int col = n0 + wn + j * 16 + fr;
             const int row = m0 + wm + i * 16 + rq;
-            if (col >= p.n || row >= p.m) {
+            if (col >= q.n || row >= q.m) {
                 continue;
             }
-            const size_t base = static_cast<size_t>(col) * p.ldc + row;
+            const size_t base = static_cast<size_t>(col) * q.ldc + row;
             float v[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 v[e] = alpha * acc[i][j][e];
             }
-            const bool full = vecC && row + 4 <= p.m;
+            const bool full = q.vecC && row + 4 <= q.m;
             if (beta != 0.0f) {
                 float c[4];
-                LoadC4<OK>(C, base, full, p.m - row, c);
+                LoadC4<OK>(q.C, base, full, q.m - row, c);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     v[e] = v[e] + beta * c[e];
@@ -553,8 +540,110 @@ __global__ __launch_bounds__(kThreads) void GemmKernel(GemmArgs p)
                     }
                 }
             }
-            StoreC4<OK>(C, base, full, p.m - row, v);
+            StoreC4<OK>(q.C, base, full, q.m - row, v);
         }
+    }
+}
+
+// One problem (or a uniform batch through pointer arrays): one workgroup per tile.
+template <int KA, int KB, int OK, int BM, int BN>
+__global__ __launch_bounds__(kThreads) void GemmKernel(GemmArgs p)
+{
+    __shared__ __attribute__((aligned(16))) uint8_t sA[2][BM * kRowBytes];
+    __shared__ __attribute__((aligned(16))) uint8_t sB[2][BN * kRowBytes];
+
+    // XCD-aware order over the whole grid, then (problem, tile) for batches
+    int bid = blockIdx.x;
+    if (gridDim.x % 8 == 0) {
+        bid = (bid % 8) * (gridDim.x / 8) + bid / 8;
+    }
+    const int tilesPer = p.tilesM * p.tilesN;
+    const int prob = bid / tilesPer;
+    int tm, tn;
+    TileOf(bid - prob * tilesPer, p.tilesM, p.tilesN, tm, tn);
+
+    TileJob q;
+    q.A = static_cast<const uint8_t*>(p.Aarr != nullptr ? p.Aarr[prob] : p.A);
+    q.B = static_cast<const uint8_t*>(p.Barr != nullptr ? p.Barr[prob] : p.B);
+    q.C = p.Carr != nullptr ? p.Carr[prob] : p.C;
+    q.m = p.m;
+    q.n = p.n;
+    q.k = p.k;
+    q.lda = p.lda;
+    q.ldb = p.ldb;
+    q.ldc = p.ldc;
+    // op(A) row r = C row; k-contiguous when A is transposed.  op(B) column c =
+    // C column; k-contiguous when B is not transposed.
+    q.aK = p.transA != 0;
+    q.bK = p.transB == 0;
+    // per-problem pointers of a batch may be less aligned than the host assumed
+    q.vecA = p.vecA && (reinterpret_cast<uintptr_t>(q.A) % 16 == 0);
+    q.vecB = p.vecB && (reinterpret_cast<uintptr_t>(q.B) % 16 == 0);
+    q.vecC = p.vecC && (reinterpret_cast<uintptr_t>(q.C) % 16 == 0);
+    float alpha = p.alpha;
+    if (p.scaleA != nullptr) alpha *= *p.scaleA;
+    if (p.scaleB != nullptr) alpha *= *p.scaleB;
+    q.alpha = alpha;
+    q.beta = p.beta;
+    GemmTile<KA, KB, OK, BM, BN>(q, tm * BM, tn * BN, sA, sB);
+}
+
+// Problem record of the persistent grouped kernel (built on the host).
+struct GroupedProblem {
+    const void* A;
+    const void* B;
+    void* C;
+    int m, n, k;
+    int lda, ldb, ldc;
+    int transA, transB;
+    float alpha, beta;
+    int tilesM, tilesN;
+    int tileStart;          // first global tile index of this problem
+    int vecA, vecB, vecC;
+};
+
+// Persistent grouped GEMM: the grid strides over the concatenated tile lists
+// of all problems (problems ordered by decreasing k on the host, so the
+// longest tiles start first); each workgroup finds its problem by binary
+// search over tileStart.
+template <int KA, int KB, int OK, int BM, int BN>
+__global__ __launch_bounds__(kThreads) void GroupedGemmKernel(const GroupedProblem* __restrict__ probs,
+                                                             int numProbs, int totalTiles)
+{
+    __shared__ __attribute__((aligned(16))) uint8_t sA[2][BM * kRowBytes];
+    __shared__ __attribute__((aligned(16))) uint8_t sB[2][BN * kRowBytes];
+    for (int tile = blockIdx.x; tile < totalTiles; tile += gridDim.x) {
+        int lo = 0;
+        int hi = numProbs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (probs[mid].tileStart <= tile) {
+                lo = mid;
+            } else {
+                hi = mid - 1;
+            }
+        }
+        const GroupedProblem& g = probs[lo];
+        int tm, tn;
+        TileOf(tile - g.tileStart, g.tilesM, g.tilesN, tm, tn);
+        TileJob q;
+        q.A = static_cast<const uint8_t*>(g.A);
+        q.B = static_cast<const uint8_t*>(g.B);
+        q.C = g.C;
+        q.m = g.m;
+        q.n = g.n;
+        q.k = g.k;
+        q.lda = g.lda;
+        q.ldb = g.ldb;
+        q.ldc = g.ldc;
+        q.aK = g.transA != 0;
+        q.bK = g.transB == 0;
+        q.vecA = g.vecA != 0;
+        q.vecB = g.vecB != 0;
+        q.vecC = g.vecC != 0;
+        q.alpha = g.alpha;
+        q.beta = g.beta;
+        GemmTile<KA, KB, OK, BM, BN>(q, tm * BM, tn * BN, sA, sB);
     }
 }
 

@@ -22,6 +22,7 @@
 // order (l = a_ik / p, a_ij = fma(-l, u_kj, a_ij)), so all three produce
 // bit-identical factors and pivots.
 #include "device_common.h"
+#include "drivers.h"
 #include "internal.h"
 #include "lapack_kernels.h"
 

@@ -18,6 +18,7 @@
 // (b_i = fma(-t_ij, x_j, b_i) for j in solve order, x_j = b_j / u_jj), so the
 // result does not depend on the blocking.
 #include "device_common.h"
+#include "drivers.h"
 #include "internal.h"
 #include "lapack_kernels.h"
 

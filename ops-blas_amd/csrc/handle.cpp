@@ -198,6 +198,43 @@ aclblasStatus_t EnsureWorkspace(_aclblas_handle* h, size_t bytes)
     return AllocateLibraryWorkspace(h, size);
 }
 
+aclblasStatus_t StageToWorkspace(_aclblas_handle* h, const void* src, size_t bytes, void** dev)
+{
+    aclblasStatus_t st = EnsureWorkspace(h, bytes);
+    if (st != ACLBLAS_STATUS_SUCCESS) {
+        return st;
+    }
+    if (h->stage_event == nullptr) {
+        if (hipEventCreateWithFlags(&h->stage_event, hipEventDisableTiming) != hipSuccess) {
+            h->stage_event = nullptr;
+            return ACLBLAS_STATUS_EXECUTION_FAILED;
+        }
+    } else if (hipEventSynchronize(h->stage_event) != hipSuccess) {
+        return ACLBLAS_STATUS_EXECUTION_FAILED;
+    }
+    if (bytes > h->host_stage_size) {
+        if (h->host_stage != nullptr) {
+            (void)hipHostFree(h->host_stage);
+            h->host_stage = nullptr;
+            h->host_stage_size = 0;
+        }
+        size_t size = bytes < 65536 ? 65536 : bytes;
+        if (hipHostMalloc(&h->host_stage, size, hipHostMallocDefault) != hipSuccess) {
+            h->host_stage = nullptr;
+            LogError("aclblasHandle", "pinned staging allocation of %zu bytes failed", size);
+            return ACLBLAS_STATUS_ALLOC_FAILED;
+        }
+        h->host_stage_size = size;
+    }
+    std::memcpy(h->host_stage, src, bytes);
+    if (hipMemcpyAsync(h->workspace, h->host_stage, bytes, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipEventRecord(h->stage_event, h->stream) != hipSuccess) {
+        return ACLBLAS_STATUS_EXECUTION_FAILED;
+    }
+    *dev = h->workspace;
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
 }  // namespace aclblas
 
 using namespace aclblas;
@@ -253,6 +290,12 @@ aclblasStatus_t aclblasDestroy(aclblasHandle_t handle)
         return ACLBLAS_STATUS_EXECUTION_FAILED;
     }
     FreeLibraryWorkspace(handle);
+    if (handle->host_stage != nullptr) {
+        (void)hipHostFree(handle->host_stage);
+    }
+    if (handle->stage_event != nullptr) {
+        (void)hipEventDestroy(handle->stage_event);
+    }
     delete handle;
     return ACLBLAS_STATUS_SUCCESS;
 }

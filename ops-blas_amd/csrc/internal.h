@@ -38,6 +38,12 @@ struct _aclblas_handle {
     // persistent grids).  device < 0 means no GPU was visible at creation.
     int device = -1;
     int cu_count = 0;
+    // Pinned host staging for per-call parameter tables (grouped GEMM), copied
+    // to the workspace with an asynchronous copy; stage_event marks the end of
+    // the last such copy so the staging memory is not rewritten under it.
+    void* host_stage = nullptr;
+    size_t host_stage_size = 0;
+    hipEvent_t stage_event = nullptr;
 };
 
 namespace aclblas {
@@ -54,6 +60,12 @@ aclblasStatus_t SyncHandleStream(_aclblas_handle* h);
 // aclblas_handle_internal.h:162-209).  Never called on the hot path: every
 // hot-path kernel here runs without a workspace.
 aclblasStatus_t EnsureWorkspace(_aclblas_handle* h, size_t bytes);
+
+// Copies `bytes` of host data to the start of the handle's workspace,
+// asynchronously on the handle's stream, through the handle's pinned staging
+// buffer; *dev receives the device address.  Blocks only while a previous
+// staging copy of this handle is still pending.
+aclblasStatus_t StageToWorkspace(_aclblas_handle* h, const void* src, size_t bytes, void** dev);
 
 // Checks that the handle can launch work and that the last launch was accepted.
 inline bool HandleHasDevice(const _aclblas_handle* h) { return h->device >= 0; }

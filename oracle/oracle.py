@@ -140,6 +140,23 @@ def getrs_batched(trans, LU, ipiv, B, nthreads=None):
     return B
 
 
+def getrs_adjoint(LU, ipiv, X, dX, nthreads=None):
+    """Reverse-mode derivative of X = A^-1 B from the LU factors of A.
+
+    dB = A^-T dX by the transposed solve (getrs_batched(1, ...)), then
+    dA = -dB X^T accumulated in FP64 and rounded once to FP32.  Shapes as in
+    getrs_batched: LU (batch, n, lda), X and dX (batch, nrhs, ld).  Returns
+    (dB like dX, dA (batch, n, n) column-major).
+    """
+    dB = getrs_batched(1, LU, ipiv, dX, nthreads)
+    n = LU.shape[1]
+    Xm = np.asarray(X, dtype=np.float64)[:, :, :n]      # (batch, nrhs, n): [b, c, i] = X_ic
+    dBm = dB.astype(np.float64)[:, :, :n]
+    # dA_ij = -sum_c dB_ic X_jc; column-major storage [b, j, i]
+    dA = -np.einsum("bcj,bci->bji", Xm, dBm)
+    return dB, dA.astype(np.float32)
+
+
 def getri_batched(LU, ipiv, ldc=None, nthreads=None):
     LU = np.ascontiguousarray(LU, dtype=np.float32)
     batch, n, lda = LU.shape

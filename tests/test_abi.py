@@ -292,3 +292,25 @@ def test_sgemm_batched_argument_checks(cpu_handle):
     assert f(cpu_handle, B.OP_N, B.OP_N, 4, 4, 4, 1.0, None, 4, DUMMY, 4, 0.0, DUMMY, 4, 1) == inv
     assert f(cpu_handle, B.OP_N, B.OP_N, 0, 4, 4, 1.0, DUMMY, 4, DUMMY, 4, 0.0, DUMMY, 4, 1) == ok
     assert f(cpu_handle, B.OP_N, B.OP_N, 4, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 4, 0) == ok
+
+
+def test_getrs_adjoint_argument_checks(cpu_handle):
+    ok = status_code("SUCCESS")
+    inv = status_code("INVALID_VALUE")
+    f = B.sgetrs_batched_adjoint
+    D = DUMMY
+    assert f(None, 4, 2, D, 4, D, D, 4, D, 4, D, 4, 1) == status_code("HANDLE_IS_NULLPTR")
+    assert f(cpu_handle, -1, 2, D, 4, D, D, 4, D, 4, D, 4, 1) == inv
+    assert f(cpu_handle, 4, -1, D, 4, D, D, 4, D, 4, D, 4, 1) == inv
+    assert f(cpu_handle, 4, 257, D, 4, D, D, 4, D, 4, D, 4, 1) == inv
+    assert f(cpu_handle, 4, 2, D, 4, D, D, 4, D, 4, D, 4, -1) == inv
+    assert f(cpu_handle, 4, 2, D, 3, D, D, 4, D, 4, D, 4, 1) == inv      # lda
+    assert f(cpu_handle, 4, 2, D, 4, D, D, 3, D, 4, D, 4, 1) == inv      # ldx
+    assert f(cpu_handle, 4, 2, D, 4, D, D, 4, D, 3, D, 4, 1) == inv      # lddx
+    assert f(cpu_handle, 4, 2, D, 4, D, D, 4, D, 4, D, 3, 1) == inv      # ldda
+    assert f(cpu_handle, 4, 2, None, 4, D, D, 4, D, 4, D, 4, 1) == inv   # LU
+    assert f(cpu_handle, 4, 2, D, 4, D, D, 4, None, 4, D, 4, 1) == inv   # dX
+    assert f(cpu_handle, 4, 2, D, 4, D, None, 4, D, 4, D, 4, 1) == inv   # X needed for dA
+    # quick returns before the device is needed
+    assert f(cpu_handle, 0, 2, None, 1, None, None, 1, None, 1, None, 1, 1) == ok
+    assert f(cpu_handle, 4, 2, None, 4, None, None, 4, None, 4, None, 4, 0) == ok
