@@ -176,6 +176,30 @@ aclblasStatus_t aclblasSgetrsBatchedAdjoint(
     const float* const Xarray[], int ldx, float* const dXarray[], int lddx, float* const dAarray[], int ldda,
     int batchCount);
 
+/* ------------------------------------------------------------------------ */
+/* Mixed-precision batched solve                                             */
+/* ------------------------------------------------------------------------ */
+
+/* Solves A_i X_i = B_i for a batch of systems of different orders (no
+ * counterpart in the reference; it follows the reference's LU semantics of
+ * blas/getrf_batched/arch35/sgetrf_batched_kernel.cpp:52-150 - partial
+ * pivoting, stop at an exact zero pivot - with a blocked algorithm).  Per
+ * system: n = nArray[i] <= 256, nrhs = nrhsArray[i] <= 16 (larger:
+ * NOT_SUPPORTED), column-major FP32 A (lda), B (ldb), X (ldx); A and B are
+ * not modified.  The LU runs in FP32 with its trailing updates on BF16 matrix
+ * cores (operands split into three BF16 terms); the solution is refined with
+ * FP64 residuals until every right-hand side has normwise backward error
+ * |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol, or maxIter corrections.
+ * nArray .. ldxArray and the pointer arrays are HOST arrays (device matrices);
+ * iterArray (corrections applied), berrArray (final worst backward error) and
+ * infoArray (0, or k+1 for an exact zero pivot at column k, X = NaN) are
+ * device arrays of batchCount entries, each may be NULL.  Works in the
+ * handle's workspace (4 n^2 + O(n^2/8) bytes per system plus a 48-byte record). */
+aclblasStatus_t aclblasSgesvBatchedMixed(
+    aclblasHandle_t handle, int batchCount, const int* nArray, const int* nrhsArray, const float* const Aarray[],
+    const int* ldaArray, const float* const Barray[], const int* ldbArray, float* const Xarray[],
+    const int* ldxArray, double tol, int maxIter, int* iterArray, float* berrArray, int* infoArray);
+
 #ifdef __cplusplus
 }
 #endif

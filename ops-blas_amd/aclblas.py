@@ -62,6 +62,7 @@ _SIGNATURES = {
     "aclblasSgetriBatched": [_P, _I, _P, _I, _P, _P, _I, _P, _I],
     "aclblasSgetrsBatchedAdjoint": [_P, _I, _I, _P, _I, _P, _P, _I, _P, _I, _P, _I, _I],
     "aclblasSgemmGroupedBatched": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "aclblasSgesvBatchedMixed": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, _I, _P, _P, _P],
 }
 
 
@@ -183,6 +184,16 @@ def sgemm_grouped_batched(h, transa, transb, m, n, k, alpha, A, lda, Bp, ldb, be
 def sgemm_grouped_batched_raw(h, group_count, *arrays):
     ptrs = [ctypes.cast(a, ctypes.c_void_p) if a is not None else None for a in arrays]
     return load().aclblasSgemmGroupedBatched(h, group_count, *ptrs)
+
+
+def sgesv_batched_mixed(h, n, nrhs, A, lda, Bp, ldb, X, ldx, tol, max_iter, iters=None, berr=None, info=None):
+    """n .. ldx: Python lists (host arrays); A/Bp/X: lists of device addresses; iters/berr/info: device addresses."""
+    count = len(n) if n is not None else 0
+    arrs = [_host_array(ctypes.c_int, n), _host_array(ctypes.c_int, nrhs), _host_array(ctypes.c_void_p, A),
+            _host_array(ctypes.c_int, lda), _host_array(ctypes.c_void_p, Bp), _host_array(ctypes.c_int, ldb),
+            _host_array(ctypes.c_void_p, X), _host_array(ctypes.c_int, ldx)]
+    ptrs = [ctypes.cast(a, ctypes.c_void_p) if a is not None else None for a in arrs]
+    return load().aclblasSgesvBatchedMixed(h, count, *ptrs, float(tol), int(max_iter), iters, berr, info)
 
 
 def sgetrs_batched_adjoint(h, n, nrhs, LUarray, lda, piv, Xarray, ldx, dXarray, lddx, dAarray, ldda, batch):

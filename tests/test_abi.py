@@ -314,3 +314,22 @@ def test_getrs_adjoint_argument_checks(cpu_handle):
     # quick returns before the device is needed
     assert f(cpu_handle, 0, 2, None, 1, None, None, 1, None, 1, None, 1, 1) == ok
     assert f(cpu_handle, 4, 2, None, 4, None, None, 4, None, 4, None, 4, 0) == ok
+
+
+def test_gesv_mixed_argument_checks(cpu_handle):
+    inv = status_code("INVALID_VALUE")
+    ns = status_code("NOT_SUPPORTED")
+    f = B.sgesv_batched_mixed
+    D = DUMMY
+    assert f(None, [4], [1], [D], [4], [D], [4], [D], [4], 1e-6, 3) == status_code("HANDLE_IS_NULLPTR")
+    assert f(cpu_handle, [4], [1], [D], [4], [D], [4], [D], [4], -1.0, 3) == inv       # tol
+    assert f(cpu_handle, [4], [1], [D], [4], [D], [4], [D], [4], 1e-6, -1) == inv      # maxIter
+    assert f(cpu_handle, [-1], [1], [D], [1], [D], [1], [D], [1], 1e-6, 3) == inv      # n
+    assert f(cpu_handle, [4], [-1], [D], [4], [D], [4], [D], [4], 1e-6, 3) == inv      # nrhs
+    assert f(cpu_handle, [4], [1], [D], [3], [D], [4], [D], [4], 1e-6, 3) == inv       # lda
+    assert f(cpu_handle, [4], [1], [D], [4], [D], [3], [D], [4], 1e-6, 3) == inv       # ldb
+    assert f(cpu_handle, [4], [1], [D], [4], [D], [4], [D], [3], 1e-6, 3) == inv       # ldx
+    assert f(cpu_handle, [4], [1], [None], [4], [D], [4], [D], [4], 1e-6, 3) == inv    # A
+    assert f(cpu_handle, [257], [1], [D], [257], [D], [257], [D], [257], 1e-6, 3) == ns
+    assert f(cpu_handle, [4], [17], [D], [4], [D], [4], [D], [4], 1e-6, 3) == ns
+    assert f(cpu_handle, [], [], [], [], [], [], [], [], 1e-6, 3) == status_code("SUCCESS")
