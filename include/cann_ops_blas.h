@@ -94,6 +94,31 @@ aclblasStatus_t aclblasGemmEx(
     const void* alpha, const void* A, aclDataType Atype, int lda, const void* B, aclDataType Btype, int ldb,
     const void* beta, void* C, aclDataType Ctype, int ldc, aclblasComputeType_t computeType, aclblasGemmAlgo_t algo);
 
+/* FP8 GEMM with per-tensor dequantisation scales (no counterpart in the
+ * reference; the scale semantics are those of its Lt attributes
+ * ACLBLASLT_MATMUL_DESC_A_SCALE_POINTER / B_SCALE_POINTER, reference
+ * include/cann_ops_blasLt.h:160-200):
+ *   C = alpha * scaleA[0] * scaleB[0] * op(A) op(B) + beta * C.
+ * Atype/Btype: ACL_FLOAT8_E4M3FN or ACL_FLOAT8_E5M2 (any mix); Ctype: ACL_FLOAT16,
+ * ACL_BF16 or ACL_FLOAT (else NOT_SUPPORTED).  alpha/beta: host floats;
+ * scaleA/scaleB: device floats, NULL meaning 1.  Runs on the native FP8 matrix
+ * instruction: each block of 128 products is summed with a relative error of
+ * about 2^-14 of its largest product (FP32 accumulation across blocks).  Bad
+ * enums or dimensions: INVALID_VALUE. */
+aclblasStatus_t aclblasGemmExScaled(
+    aclblasHandle_t handle, aclblasOperation_t transA, aclblasOperation_t transB, int m, int n, int k,
+    const float* alpha, const void* A, aclDataType Atype, int lda, const float* scaleA, const void* B,
+    aclDataType Btype, int ldb, const float* scaleB, const float* beta, void* C, aclDataType Ctype, int ldc);
+
+/* Per-tensor FP8 quantisation (no counterpart in the reference; formats and
+ * rounding of reference test/frame/dtype_utils.h:39-160): *scale = amax(|x|) /
+ * fp8_max (1 when x is all zero; fp8_max = 448 for E4M3FN, 57344 for E5M2) and
+ * q[i] = fp8(x[i] * (fp8_max / amax)), round to nearest even, saturating.  x:
+ * count device floats; q: count device bytes; scale: one device float, written
+ * on the handle's stream (so it can feed aclblasGemmExScaled directly). */
+aclblasStatus_t aclblasQuantizeFp8(aclblasHandle_t handle, long long count, const float* x, aclDataType qtype,
+                                   void* q, float* scale);
+
 /* reference include/cann_ops_blas.h:409-414.  C[i] = alpha*op(A[i])*op(B[i]) +
  * beta*C[i] over device arrays of device pointers, one shape for the batch.
  * Accepted types: FP16 (COMPUTE_16F or 32F), BF16 and FP8 E4M3/E5M2 (any mix,

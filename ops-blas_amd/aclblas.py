@@ -63,6 +63,8 @@ _SIGNATURES = {
     "aclblasSgetrsBatchedAdjoint": [_P, _I, _I, _P, _I, _P, _P, _I, _P, _I, _P, _I, _I],
     "aclblasSgemmGroupedBatched": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "aclblasSgesvBatchedMixed": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, _I, _P, _P, _P],
+    "aclblasGemmExScaled": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _P, _P, _I, _I, _P, _P, _P, _I, _I],
+    "aclblasQuantizeFp8": [_P, ctypes.c_longlong, _P, _I, _P, _P],
 }
 
 
@@ -194,6 +196,16 @@ def sgesv_batched_mixed(h, n, nrhs, A, lda, Bp, ldb, X, ldx, tol, max_iter, iter
             _host_array(ctypes.c_void_p, X), _host_array(ctypes.c_int, ldx)]
     ptrs = [ctypes.cast(a, ctypes.c_void_p) if a is not None else None for a in arrs]
     return load().aclblasSgesvBatchedMixed(h, count, *ptrs, float(tol), int(max_iter), iters, berr, info)
+
+
+def gemm_ex_scaled(h, ta, tb, m, n, k, alpha, A, atype, lda, scale_a, B, btype, ldb, scale_b, beta, C, ctype, ldc):
+    """scale_a / scale_b: device addresses of one float each (or None)."""
+    return load().aclblasGemmExScaled(h, ta, tb, m, n, k, _fp(alpha), A, atype, lda, scale_a, B, btype, ldb, scale_b,
+                                      _fp(beta), C, ctype, ldc)
+
+
+def quantize_fp8(h, count, x, qtype, q, scale):
+    return load().aclblasQuantizeFp8(h, count, x, qtype, q, scale)
 
 
 def sgetrs_batched_adjoint(h, n, nrhs, LUarray, lda, piv, Xarray, ldx, dXarray, lddx, dAarray, ldda, batch):

@@ -93,33 +93,6 @@ void LaunchScaleBatched(hipStream_t st, int outKind, void* const* Carr, int m, i
     }
 }
 
-// Tile shape per problem: 256 x 256 for large 1- and 2-byte GEMMs (64 KiB LDS
-// per stage), 128 x 128 for large FP32 ones, 64 x 64 when the 128-tile grid
-// would leave compute units idle.
-template <int KA, int KB, int OK>
-void LaunchGemmTyped(hipStream_t st, GemmArgs p, int batch, int cus)
-{
-    constexpr int EB = KindInfo<KA>::bytes;
-    const long long t256 = static_cast<long long>((p.m + 255) / 256) * ((p.n + 255) / 256) * batch;
-    const long long t128 = static_cast<long long>((p.m + 127) / 128) * ((p.n + 127) / 128) * batch;
-    if (EB <= 2 && t256 >= cus) {
-        p.tilesM = (p.m + 255) / 256;
-        p.tilesN = (p.n + 255) / 256;
-        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 256, 256>), dim3(p.tilesM * p.tilesN * batch), dim3(kThreads),
-                           0, st, p);
-    } else if (t128 >= cus) {
-        p.tilesM = (p.m + 127) / 128;
-        p.tilesN = (p.n + 127) / 128;
-        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 128, 128>), dim3(p.tilesM * p.tilesN * batch), dim3(kThreads),
-                           0, st, p);
-    } else {
-        p.tilesM = (p.m + 63) / 64;
-        p.tilesN = (p.n + 63) / 64;
-        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 64, 64>), dim3(p.tilesM * p.tilesN * batch), dim3(kThreads), 0,
-                           st, p);
-    }
-}
-
 bool Aligned16(const void* p, int ld, int eb)
 {
     return (reinterpret_cast<uintptr_t>(p) % 16 == 0) && ((static_cast<long long>(ld) * eb) % 16 == 0);

@@ -35,8 +35,10 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 
-// Element kinds (also the LDS byte layouts).
-enum Kind { kF32 = 0, kBF16 = 1, kF16 = 2, kE4M3 = 3, kE5M2 = 4 };
+// Element kinds (also the LDS byte layouts).  kE4M3/kE5M2: FP8 operands with
+// FP32-grade accumulation (widened to BF16); kE4M3S/kE5M2S: FP8 operands on the
+// native FP8 matrix instruction (the per-tensor-scaled entry).
+enum Kind { kF32 = 0, kBF16 = 1, kF16 = 2, kE4M3 = 3, kE5M2 = 4, kE4M3S = 5, kE5M2S = 6 };
 
 template <int K>
 struct KindInfo;
@@ -58,6 +60,14 @@ struct KindInfo<kE4M3> {
 };
 template <>
 struct KindInfo<kE5M2> {
+    static constexpr int bytes = 1;
+};
+template <>
+struct KindInfo<kE4M3S> {
+    static constexpr int bytes = 1;
+};
+template <>
+struct KindInfo<kE5M2S> {
     static constexpr int bytes = 1;
 };
 
@@ -314,6 +324,33 @@ template <>
 struct Mfma<kE5M2, kE4M3> : Fp8ViaBf16Mfma<kE5M2, kE4M3> {};
 template <>
 struct Mfma<kE5M2, kE5M2> : Fp8ViaBf16Mfma<kE5M2, kE5M2> {};
+
+// Native FP8: one v_mfma_f32_16x16x128_f8f6f4 per 32-byte fragment.  Operand
+// format codes (cbsz / blgp): 0 = E4M3, 1 = E5M2.  Its 128 products are summed
+// in a narrower window than FP32 (relative error about 2^-14 of the largest
+// product, measured), which the scaled entry states as its accuracy.
+template <int KA, int KB>
+struct Fp8NativeMfma {
+    static constexpr int fa = KA == kE5M2S ? 1 : 0;
+    static constexpr int fb = KB == kE5M2S ? 1 : 0;
+    __device__ __forceinline__ static void Run(const uint4 (&a)[2], const uint4 (&b)[2], f32x4& acc)
+    {
+        struct P {
+            uint4 x, y;
+        };
+        const i32x8 va = __builtin_bit_cast(i32x8, P{a[0], a[1]});
+        const i32x8 vb = __builtin_bit_cast(i32x8, P{b[0], b[1]});
+        acc = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(va, vb, acc, fa, fb, 0, 0, 0, 0);
+    }
+};
+template <>
+struct Mfma<kE4M3S, kE4M3S> : Fp8NativeMfma<kE4M3S, kE4M3S> {};
+template <>
+struct Mfma<kE4M3S, kE5M2S> : Fp8NativeMfma<kE4M3S, kE5M2S> {};
+template <>
+struct Mfma<kE5M2S, kE4M3S> : Fp8NativeMfma<kE5M2S, kE4M3S> {};
+template <>
+struct Mfma<kE5M2S, kE5M2S> : Fp8NativeMfma<kE5M2S, kE5M2S> {};
 
 // Output element conversions (round to nearest even).
 __device__ __forceinline__ uint16_t ToBf16(float f)
@@ -644,6 +681,33 @@ __global__ __launch_bounds__(kThreads) void GroupedGemmKernel(const GroupedProbl
         q.alpha = g.alpha;
         q.beta = g.beta;
         GemmTile<KA, KB, OK, BM, BN>(q, tm * BM, tn * BN, sA, sB);
+    }
+}
+
+// Tile shape per problem: 256 x 256 for large 1- and 2-byte GEMMs (64 KiB LDS
+// per stage), 128 x 128 for large FP32 ones, 64 x 64 when the 128-tile grid
+// would leave compute units idle.
+template <int KA, int KB, int OK>
+inline void LaunchGemmTyped(hipStream_t st, GemmArgs p, int batch, int cus)
+{
+    constexpr int EB = KindInfo<KA>::bytes;
+    const long long t256 = static_cast<long long>((p.m + 255) / 256) * ((p.n + 255) / 256) * batch;
+    const long long t128 = static_cast<long long>((p.m + 127) / 128) * ((p.n + 127) / 128) * batch;
+    if (EB <= 2 && t256 >= cus) {
+        p.tilesM = (p.m + 255) / 256;
+        p.tilesN = (p.n + 255) / 256;
+        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 256, 256>), dim3(p.tilesM * p.tilesN * batch), dim3(kThreads),
+                           0, st, p);
+    } else if (t128 >= cus) {
+        p.tilesM = (p.m + 127) / 128;
+        p.tilesN = (p.n + 127) / 128;
+        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 128, 128>), dim3(p.tilesM * p.tilesN * batch), dim3(kThreads),
+                           0, st, p);
+    } else {
+        p.tilesM = (p.m + 63) / 64;
+        p.tilesN = (p.n + 63) / 64;
+        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 64, 64>), dim3(p.tilesM * p.tilesN * batch), dim3(kThreads), 0,
+                           st, p);
     }
 }
 

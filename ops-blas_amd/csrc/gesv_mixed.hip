@@ -51,7 +51,6 @@ constexpr int kMaxRhs = 16;
 constexpr int kNb = 32;                  // panel width
 constexpr int kThreads = 256;            // four waves
 constexpr int kWaves = kThreads / kWave;
-constexpr int kSlots = kMaxN / kWave;    // panel rows per lane
 constexpr int kTileN = 16;               // sweep tile width
 constexpr int kMaxTiles = kMaxN / 16;    // 16-row accumulator tiles per column tile
 constexpr int kLStride = kNb;            // bf16 elements per L21 plane row (64 bytes, 16-byte aligned)

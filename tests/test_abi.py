@@ -333,3 +333,27 @@ def test_gesv_mixed_argument_checks(cpu_handle):
     assert f(cpu_handle, [257], [1], [D], [257], [D], [257], [D], [257], 1e-6, 3) == ns
     assert f(cpu_handle, [4], [17], [D], [4], [D], [4], [D], [4], 1e-6, 3) == ns
     assert f(cpu_handle, [], [], [], [], [], [], [], [], 1e-6, 3) == status_code("SUCCESS")
+
+
+def test_gemm_ex_scaled_and_quantizer_argument_checks(cpu_handle):
+    inv = status_code("INVALID_VALUE")
+    ns = status_code("NOT_SUPPORTED")
+    e4, e5 = B.DTYPES["FP8_E4M3FN"], B.DTYPES["FP8_E5M2"]
+    f = B.gemm_ex_scaled
+    D = DUMMY
+    assert f(None, B.OP_N, B.OP_N, 4, 4, 4, 1.0, D, e4, 4, D, D, e4, 4, D, 0.0, D, B.ACL_FLOAT16, 4) == \
+        status_code("HANDLE_IS_NULLPTR")
+    assert f(cpu_handle, 999, B.OP_N, 4, 4, 4, 1.0, D, e4, 4, D, D, e4, 4, D, 0.0, D, B.ACL_FLOAT16, 4) == inv
+    assert f(cpu_handle, B.OP_N, B.OP_N, -1, 4, 4, 1.0, D, e4, 4, D, D, e4, 4, D, 0.0, D, B.ACL_FLOAT16, 4) == inv
+    assert f(cpu_handle, B.OP_N, B.OP_N, 4, 4, 4, 1.0, D, e4, 3, D, D, e4, 4, D, 0.0, D, B.ACL_FLOAT16, 4) == inv
+    assert f(cpu_handle, B.OP_N, B.OP_N, 4, 4, 4, None, D, e4, 4, D, D, e4, 4, D, 0.0, D, B.ACL_FLOAT16, 4) == inv
+    assert f(cpu_handle, B.OP_N, B.OP_N, 4, 4, 4, 1.0, D, B.ACL_BF16, 4, D, D, e4, 4, D, 0.0, D, B.ACL_FLOAT16,
+             4) == ns
+    assert f(cpu_handle, B.OP_N, B.OP_N, 4, 4, 4, 1.0, D, e4, 4, D, D, e5, 4, D, 0.0, D, 2, 4) == ns
+    assert f(cpu_handle, B.OP_N, B.OP_N, 0, 4, 4, 1.0, D, e4, 4, D, D, e5, 4, D, 0.0, D, B.ACL_FLOAT16, 4) == \
+        status_code("SUCCESS")
+    q = B.quantize_fp8
+    assert q(None, 4, D, e4, D, D) == status_code("HANDLE_IS_NULLPTR")
+    assert q(cpu_handle, -1, D, e4, D, D) == inv
+    assert q(cpu_handle, 4, D, B.ACL_BF16, D, D) == inv
+    assert q(cpu_handle, 4, D, e4, D, None) == inv
