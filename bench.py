@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--primary-only", action="store_true", help="skip the secondary workloads")
+    ap.add_argument("--profile-gemm", action="store_true",
+                    help="only run three BF16 x BF16 -> FP32 8192^3 GEMMs (for counter collection) and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="before warm-up, run 5 elementwise negations of a MIB-MiB FP32 tensor (read and write "
                          "MIB MiB each): the known-size reference that scales hardware byte counters")
@@ -302,6 +304,21 @@ def main():
     B.load()
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.profile_gemm:
+        h = B.Handle()
+        h.set_stream(torch.cuda.current_stream().cuda_stream)
+        m = 8192
+        a = torch.randn(m * m, device=dev).to(torch.bfloat16)
+        b = torch.randn(m * m, device=dev).to(torch.bfloat16)
+        c = torch.zeros(m * m, device=dev)
+        for _ in range(3):
+            st = B.gemm_ex(h.value, B.OP_N, B.OP_N, m, m, m, 1.0, a.data_ptr(), B.ACL_BF16, m, b.data_ptr(),
+                           B.ACL_BF16, m, 0.0, c.data_ptr(), B.ACL_FLOAT, m, B.COMPUTE["COMPUTE_32F"])
+            assert st == 0, st
+        torch.cuda.synchronize()
+        h.destroy()
+        print("gemm profile run done")
+        return
     A, Bv = W.lu_batch(BATCH, N, NRHS, seed=0, shift=4.0)
     total = args.warmup + args.steps
     dA0 = torch.from_numpy(A).to(dev)
