@@ -72,7 +72,8 @@ struct KindInfo<kE5M2S> {
 };
 
 constexpr int kRowBytes = 128;   // one k-block per LDS row
-constexpr int kThreads = 256;
+constexpr int kThreads = 256;     // default workgroup: 2 x 2 waves
+constexpr int kWaveSize = 64;
 
 struct GemmArgs {
     const void* A;
@@ -110,16 +111,108 @@ __device__ __forceinline__ uint32_t SwizzledOffset(int row, int byteInRow)
 // loads KB 16-byte chunks of E rows at KB consecutive k, transposes the E x KB
 // block in registers and stores E row segments of KB elements (2- and 4-byte
 // types; FP8 falls back to element stores).
-template <int ROWS, int EB>
+template <int ROWS, int EB, int NT = kThreads>
 struct TileLoader {
     static constexpr int kElems = kRowBytes / EB;                 // k per block
     static constexpr int kChunks = ROWS * kRowBytes / 16;         // 16-byte chunks per tile
-    static constexpr int kPerThread = kChunks / kThreads;         // = KB
+    static constexpr int kPerThread = kChunks / NT;               // = KB
     static constexpr int E = 16 / EB;                             // elements per chunk
     static constexpr int kMChunks = ROWS / E;                     // row chunks (transposed path)
-    static_assert(kChunks % kThreads == 0, "tile must split evenly");
-    static_assert(kMChunks * (kElems / kPerThread) == kThreads, "transposed blocks cover the tile");
+    static_assert(kChunks % NT == 0, "tile must split evenly");
+    static_assert(kMChunks * (kElems / kPerThread) == NT, "transposed blocks cover the tile");
+    static constexpr int kWords = kPerThread * EB / 4;  // 32-bit words per transposed row segment
+    static constexpr int kSegBlocks = kWords >= 4 ? kWords / 4 : 1;
     uint4 v[kPerThread];
+    // addressing of the unchecked path (interior tiles), fixed for one tile
+    const uint8_t* base;                 // this thread's chunk 0 at k-block 0
+    size_t qstride;                      // bytes between this thread's consecutive chunks
+    size_t kstep;                        // bytes per k-block
+    uint32_t soff[kPerThread];           // LDS offsets of the k-contiguous stores
+    uint32_t toff[E][kSegBlocks];        // LDS offsets of the transposed row segments
+
+
+    __device__ __forceinline__ void Setup(const uint8_t* X, int ld, bool kcontig, int r0)
+    {
+        const int t = threadIdx.x;
+        if (kcontig) {
+            const int r = t / (kRowBytes / 16);
+            const int kc = t % (kRowBytes / 16);
+            base = X + (static_cast<size_t>(r0 + r) * ld + kc * E) * EB;
+            qstride = static_cast<size_t>(NT / (kRowBytes / 16)) * ld * EB;
+            kstep = static_cast<size_t>(kElems) * EB;
+#pragma unroll
+            for (int q = 0; q < kPerThread; ++q) {
+                const int id = t + NT * q;
+                soff[q] = SwizzledOffset(id / (kRowBytes / 16), (id % (kRowBytes / 16)) * 16);
+            }
+        } else {
+            const int mc = t % kMChunks;
+            const int kg = t / kMChunks;
+            base = X + (static_cast<size_t>(kg * kPerThread) * ld + r0 + mc * E) * EB;
+            qstride = static_cast<size_t>(ld) * EB;
+            kstep = static_cast<size_t>(kElems) * ld * EB;
+            const int byte0 = kg * kPerThread * EB;
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+#pragma unroll
+                for (int b = 0; b < kSegBlocks; ++b) {
+                    toff[i][b] = SwizzledOffset(mc * E + i, byte0 + b * 16);
+                }
+            }
+        }
+    }
+
+    // All chunks of k-block kb in range and 16-byte aligned.
+    __device__ __forceinline__ void FetchFull(int kb)
+    {
+        const uint8_t* p = base + static_cast<size_t>(kb) * kstep;
+#pragma unroll
+        for (int q = 0; q < kPerThread; ++q) {
+            v[q] = *reinterpret_cast<const uint4*>(p + q * qstride);
+        }
+    }
+
+    // Store with the offsets of Setup (EB 2 and 4 use the register transpose).
+    __device__ __forceinline__ void StoreFast(uint8_t* lds, bool kcontig) const
+    {
+        if (kcontig) {
+#pragma unroll
+            for (int q = 0; q < kPerThread; ++q) {
+                *reinterpret_cast<uint4*>(lds + soff[q]) = v[q];
+            }
+            return;
+        }
+        if constexpr (EB == 4 || EB == 2) {
+            const uint32_t* w = reinterpret_cast<const uint32_t*>(v);
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                uint32_t seg[kWords];
+#pragma unroll
+                for (int o = 0; o < kWords; ++o) {
+                    if constexpr (EB == 4) {
+                        seg[o] = w[o * 4 + i];
+                    } else {
+                        const uint32_t lo = w[(2 * o) * 4 + i / 2];
+                        const uint32_t hi = w[(2 * o + 1) * 4 + i / 2];
+                        seg[o] = __builtin_amdgcn_perm(hi, lo, (i & 1) ? 0x07060302u : 0x05040100u);
+                    }
+                }
+                if constexpr (kWords >= 4) {
+#pragma unroll
+                    for (int o = 0; o < kWords; o += 4) {
+                        *reinterpret_cast<uint4*>(lds + toff[i][o / 4]) =
+                            make_uint4(seg[o], seg[o + 1], seg[o + 2], seg[o + 3]);
+                    }
+                } else if constexpr (kWords == 2) {
+                    *reinterpret_cast<uint2*>(lds + toff[i][0]) = make_uint2(seg[0], seg[1]);
+                } else {
+                    *reinterpret_cast<uint32_t*>(lds + toff[i][0]) = seg[0];
+                }
+            }
+        } else {
+            Store(lds, kcontig);
+        }
+    }
 
     __device__ __forceinline__ static uint32_t LoadElem(const uint8_t* p)
     {
@@ -150,7 +243,7 @@ struct TileLoader {
         if (kcontig) {
 #pragma unroll
             for (int q = 0; q < kPerThread; ++q) {
-                const int id = t + kThreads * q;
+                const int id = t + NT * q;
                 const int r = id / (kRowBytes / 16);
                 const int gr = r0 + r;
                 const int gk = k0 + (id % (kRowBytes / 16)) * E;
@@ -178,7 +271,7 @@ struct TileLoader {
         if (kcontig) {
 #pragma unroll
             for (int q = 0; q < kPerThread; ++q) {
-                const int id = t + kThreads * q;
+                const int id = t + NT * q;
                 const int r = id / (kRowBytes / 16);
                 const int b = (id % (kRowBytes / 16)) * 16;
                 *reinterpret_cast<uint4*>(lds + SwizzledOffset(r, b)) = v[q];
@@ -236,40 +329,55 @@ struct TileLoader {
 template <int KA, int KB>
 struct Mfma;
 
+// Run: one k-block quarter (32 bytes per lane); RunHalf: its first or second
+// 16 bytes (kSplit: the instruction sequence splits at 16 bytes, so a k-block
+// can be consumed half by half with the same per-accumulator order).
 template <>
 struct Mfma<kF32, kF32> {
-    __device__ __forceinline__ static void Run(const uint4 (&a)[2], const uint4 (&b)[2], f32x4& acc)
+    static constexpr bool kSplit = true;
+    __device__ __forceinline__ static void RunHalf(const uint4& a, const uint4& b, f32x4& acc)
     {
-        const float* fa = reinterpret_cast<const float*>(a);
-        const float* fb = reinterpret_cast<const float*>(b);
+        const float fa[4] = {__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), __uint_as_float(a.w)};
+        const float fb[4] = {__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w)};
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
+        for (int s = 0; s < 4; ++s) {
             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[s], fb[s], acc, 0, 0, 0);
         }
+    }
+    __device__ __forceinline__ static void Run(const uint4 (&a)[2], const uint4 (&b)[2], f32x4& acc)
+    {
+        RunHalf(a[0], b[0], acc);
+        RunHalf(a[1], b[1], acc);
     }
 };
 
 template <>
 struct Mfma<kBF16, kBF16> {
+    static constexpr bool kSplit = true;
+    __device__ __forceinline__ static void RunHalf(const uint4& a, const uint4& b, f32x4& acc)
+    {
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                      acc, 0, 0, 0);
+    }
     __device__ __forceinline__ static void Run(const uint4 (&a)[2], const uint4 (&b)[2], f32x4& acc)
     {
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[s]),
-                                                          __builtin_bit_cast(bf16x8, b[s]), acc, 0, 0, 0);
-        }
+        RunHalf(a[0], b[0], acc);
+        RunHalf(a[1], b[1], acc);
     }
 };
 
 template <>
 struct Mfma<kF16, kF16> {
+    static constexpr bool kSplit = true;
+    __device__ __forceinline__ static void RunHalf(const uint4& a, const uint4& b, f32x4& acc)
+    {
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), acc,
+                                                     0, 0, 0);
+    }
     __device__ __forceinline__ static void Run(const uint4 (&a)[2], const uint4 (&b)[2], f32x4& acc)
     {
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a[s]),
-                                                         __builtin_bit_cast(f16x8, b[s]), acc, 0, 0, 0);
-        }
+        RunHalf(a[0], b[0], acc);
+        RunHalf(a[1], b[1], acc);
     }
 };
 
@@ -305,15 +413,18 @@ __device__ __forceinline__ bf16x8 Fp8x8ToBf16(uint32_t w0, uint32_t w1)
 
 template <int KA, int KB>
 struct Fp8ViaBf16Mfma {
+    static constexpr bool kSplit = true;
+    __device__ __forceinline__ static void RunHalf(const uint4& a, const uint4& b, f32x4& acc)
+    {
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Fp8x8ToBf16<KA>(a.x, a.y), Fp8x8ToBf16<KB>(b.x, b.y), acc, 0,
+                                                      0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Fp8x8ToBf16<KA>(a.z, a.w), Fp8x8ToBf16<KB>(b.z, b.w), acc, 0,
+                                                      0, 0);
+    }
     __device__ __forceinline__ static void Run(const uint4 (&a)[2], const uint4 (&b)[2], f32x4& acc)
     {
-        const uint32_t* wa = reinterpret_cast<const uint32_t*>(a);
-        const uint32_t* wb = reinterpret_cast<const uint32_t*>(b);
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Fp8x8ToBf16<KA>(wa[2 * s], wa[2 * s + 1]),
-                                                          Fp8x8ToBf16<KB>(wb[2 * s], wb[2 * s + 1]), acc, 0, 0, 0);
-        }
+        RunHalf(a[0], b[0], acc);
+        RunHalf(a[1], b[1], acc);
     }
 };
 template <>
@@ -331,6 +442,8 @@ struct Mfma<kE5M2, kE5M2> : Fp8ViaBf16Mfma<kE5M2, kE5M2> {};
 // product, measured), which the scaled entry states as its accuracy.
 template <int KA, int KB>
 struct Fp8NativeMfma {
+    static constexpr bool kSplit = false;
+    __device__ __forceinline__ static void RunHalf(const uint4&, const uint4&, f32x4&) {}
     static constexpr int fa = KA == kE5M2S ? 1 : 0;
     static constexpr int fb = KB == kE5M2S ? 1 : 0;
     __device__ __forceinline__ static void Run(const uint4 (&a)[2], const uint4 (&b)[2], f32x4& acc)
@@ -468,16 +581,17 @@ struct TileJob {
 // with the LDS stages sA/sB.  FP8 non-finite results become NaN (reference
 // test/gemm/gemm_ex/gemm_ex_golden.h, QuantizeGoldenOutput: the reference's FP8
 // path yields NaN for non-finite).  Ends with the LDS free for the next tile.
-template <int KA, int KB, int OK, int BM, int BN>
+template <int KA, int KB, int OK, int BM, int BN, int WGM = 2, int WGN = 2>
 __device__ __forceinline__ void GemmTile(const TileJob& q, int m0, int n0, uint8_t (*sA)[BM * kRowBytes],
                                          uint8_t (*sB)[BN * kRowBytes])
 {
+    constexpr int NT = kWaveSize * WGM * WGN;
     constexpr int EA = KindInfo<KA>::bytes;
     constexpr int EB = KindInfo<KB>::bytes;
     static_assert(EA == EB, "operands share the k-block size");
     constexpr int kK = kRowBytes / EA;   // k per block
-    constexpr int WM = BM / 2;           // wave tile
-    constexpr int WN = BN / 2;
+    constexpr int WM = BM / WGM;         // wave tile
+    constexpr int WN = BN / WGN;
     constexpr int TM = WM / 16;          // MFMA tiles per wave
     constexpr int TN = WN / 16;
     constexpr bool kFp8 = (KA == kE4M3 || KA == kE5M2);
@@ -485,11 +599,17 @@ __device__ __forceinline__ void GemmTile(const TileJob& q, int m0, int n0, uint8
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wave = t >> 6;
-    const int wm = (wave & 1) * WM;
-    const int wn = (wave >> 1) * WN;
+    const int wm = (wave % WGM) * WM;
+    const int wn = (wave / WGM) * WN;
 
-    TileLoader<BM, EA> la;
-    TileLoader<BN, EB> lb;
+    TileLoader<BM, EA, NT> la;
+    TileLoader<BN, EB, NT> lb;
+    la.Setup(q.A, q.lda, q.aK, m0);
+    lb.Setup(q.B, q.ldb, q.bK, n0);
+    // interior tiles take the unchecked path for every full k-block
+    const int fullK = q.k / kK;
+    const bool interiorA = q.vecA && m0 + BM <= q.m;
+    const bool interiorB = q.vecB && n0 + BN <= q.n;
 
     f32x4 acc[TM][TN];
 #pragma unroll
@@ -498,45 +618,86 @@ __device__ __forceinline__ void GemmTile(const TileJob& q, int m0, int n0, uint8
         for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     const int nk = (q.k + kK - 1) / kK;
+    auto fetch = [&](int kb) {
+        if (interiorA && kb < fullK) {
+            la.FetchFull(kb);
+        } else {
+            la.Fetch(q.A, q.lda, q.aK, q.vecA, m0, q.m, kb * kK, q.k);
+        }
+        if (interiorB && kb < fullK) {
+            lb.FetchFull(kb);
+        } else {
+            lb.Fetch(q.B, q.ldb, q.bK, q.vecB, n0, q.n, kb * kK, q.k);
+        }
+    };
     if (nk > 0) {
-        la.Fetch(q.A, q.lda, q.aK, q.vecA, m0, q.m, 0, q.k);
-        lb.Fetch(q.B, q.ldb, q.bK, q.vecB, n0, q.n, 0, q.k);
-        la.Store(sA[0], q.aK);
-        lb.Store(sB[0], q.bK);
+        fetch(0);
+        la.StoreFast(sA[0], q.aK);
+        lb.StoreFast(sB[0], q.bK);
     }
     __syncthreads();
 
     const int fr = lane & 15;         // fragment row / column
     const int fq = (lane >> 4) * 32;  // byte offset of this lane's quarter
+    // Fragment rows r = w + 16 i + fr: the swizzle key (r ^ r >> 3) & 7 depends
+    // on i only through i mod 4, so four offsets per operand and half-row cover
+    // every fragment (plus 16 rows * 128 bytes per i).
+    // The second 16 bytes of a quarter sit at offset ^ 16 (fq / 16 is even).
+    uint32_t offA[4], offB[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        offA[i] = SwizzledOffset(wm + i * 16 + fr, fq) - i * 16 * kRowBytes;
+        offB[i] = SwizzledOffset(wn + i * 16 + fr, fq) - i * 16 * kRowBytes;
+    }
 
     for (int kb = 0; kb < nk; ++kb) {
         const int cur = kb & 1;
         const bool more = kb + 1 < nk;
         if (more) {
-            la.Fetch(q.A, q.lda, q.aK, q.vecA, m0, q.m, (kb + 1) * kK, q.k);
-            lb.Fetch(q.B, q.ldb, q.bK, q.vecB, n0, q.n, (kb + 1) * kK, q.k);
+            fetch(kb + 1);
         }
-        uint4 fb[TN][2];
+        const uint8_t* a = sA[cur];
+        const uint8_t* b = sB[cur];
+        if constexpr (Mfma<KA, KB>::kSplit) {
+            // half-quarter by half-quarter: half the fragment registers live
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            const int row = wn + j * 16 + fr;
-            fb[j][0] = *reinterpret_cast<const uint4*>(sB[cur] + SwizzledOffset(row, fq));
-            fb[j][1] = *reinterpret_cast<const uint4*>(sB[cur] + SwizzledOffset(row, fq + 16));
-        }
+            for (int h = 0; h < 2; ++h) {
+                uint4 fb[TN];
 #pragma unroll
-        for (int i = 0; i < TM; ++i) {
-            const int row = wm + i * 16 + fr;
-            uint4 fa[2];
-            fa[0] = *reinterpret_cast<const uint4*>(sA[cur] + SwizzledOffset(row, fq));
-            fa[1] = *reinterpret_cast<const uint4*>(sA[cur] + SwizzledOffset(row, fq + 16));
+                for (int j = 0; j < TN; ++j) {
+                    fb[j] = *reinterpret_cast<const uint4*>(b + ((offB[j % 4] ^ (16u * h)) + j * 16 * kRowBytes));
+                }
+#pragma unroll
+                for (int i = 0; i < TM; ++i) {
+                    const uint4 fa = *reinterpret_cast<const uint4*>(a + ((offA[i % 4] ^ (16u * h)) +
+                                                                          i * 16 * kRowBytes));
+#pragma unroll
+                    for (int j = 0; j < TN; ++j) {
+                        Mfma<KA, KB>::RunHalf(fa, fb[j], acc[i][j]);
+                    }
+                }
+            }
+        } else {
+            uint4 fb[TN][2];
 #pragma unroll
             for (int j = 0; j < TN; ++j) {
-                Mfma<KA, KB>::Run(fa, fb[j], acc[i][j]);
+                fb[j][0] = *reinterpret_cast<const uint4*>(b + (offB[j % 4] + j * 16 * kRowBytes));
+                fb[j][1] = *reinterpret_cast<const uint4*>(b + ((offB[j % 4] ^ 16u) + j * 16 * kRowBytes));
+            }
+#pragma unroll
+            for (int i = 0; i < TM; ++i) {
+                uint4 fa[2];
+                fa[0] = *reinterpret_cast<const uint4*>(a + (offA[i % 4] + i * 16 * kRowBytes));
+                fa[1] = *reinterpret_cast<const uint4*>(a + ((offA[i % 4] ^ 16u) + i * 16 * kRowBytes));
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    Mfma<KA, KB>::Run(fa, fb[j], acc[i][j]);
+                }
             }
         }
         if (more) {
-            la.Store(sA[cur ^ 1], q.aK);
-            lb.Store(sB[cur ^ 1], q.bK);
+            la.StoreFast(sA[cur ^ 1], q.aK);
+            lb.StoreFast(sB[cur ^ 1], q.bK);
         }
         __syncthreads();
     }
@@ -582,9 +743,10 @@ __device__ __forceinline__ void GemmTile(const TileJob& q, int m0, int n0, uint8
     }
 }
 
-// One problem (or a uniform batch through pointer arrays): one workgroup per tile.
-template <int KA, int KB, int OK, int BM, int BN>
-__global__ __launch_bounds__(kThreads) void GemmKernel(GemmArgs p)
+// One problem (or a uniform batch through pointer arrays): one workgroup per
+// tile, WGM x WGN waves.
+template <int KA, int KB, int OK, int BM, int BN, int WGM = 2, int WGN = 2>
+__global__ __launch_bounds__(kWaveSize * WGM * WGN) void GemmKernel(GemmArgs p)
 {
     __shared__ __attribute__((aligned(16))) uint8_t sA[2][BM * kRowBytes];
     __shared__ __attribute__((aligned(16))) uint8_t sB[2][BN * kRowBytes];
@@ -622,7 +784,7 @@ __global__ __launch_bounds__(kThreads) void GemmKernel(GemmArgs p)
     if (p.scaleB != nullptr) alpha *= *p.scaleB;
     q.alpha = alpha;
     q.beta = p.beta;
-    GemmTile<KA, KB, OK, BM, BN>(q, tm * BM, tn * BN, sA, sB);
+    GemmTile<KA, KB, OK, BM, BN, WGM, WGN>(q, tm * BM, tn * BN, sA, sB);
 }
 
 // Problem record of the persistent grouped kernel (built on the host).
@@ -685,8 +847,9 @@ __global__ __launch_bounds__(kThreads) void GroupedGemmKernel(const GroupedProbl
 }
 
 // Tile shape per problem: 256 x 256 for large 1- and 2-byte GEMMs (64 KiB LDS
-// per stage), 128 x 128 for large FP32 ones, 64 x 64 when the 128-tile grid
-// would leave compute units idle.
+// per stage, 8 waves of 128 x 64 so that two waves share each SIMD), 128 x 128
+// (4 waves) for large FP32 ones, 64 x 64 when the 128-tile grid would leave
+// compute units idle.
 template <int KA, int KB, int OK>
 inline void LaunchGemmTyped(hipStream_t st, GemmArgs p, int batch, int cus)
 {
@@ -696,8 +859,8 @@ inline void LaunchGemmTyped(hipStream_t st, GemmArgs p, int batch, int cus)
     if (EB <= 2 && t256 >= cus) {
         p.tilesM = (p.m + 255) / 256;
         p.tilesN = (p.n + 255) / 256;
-        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 256, 256>), dim3(p.tilesM * p.tilesN * batch), dim3(kThreads),
-                           0, st, p);
+        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 256, 256, 2, 4>), dim3(p.tilesM * p.tilesN * batch),
+                           dim3(kWaveSize * 8), 0, st, p);
     } else if (t128 >= cus) {
         p.tilesM = (p.m + 127) / 128;
         p.tilesN = (p.n + 127) / 128;
