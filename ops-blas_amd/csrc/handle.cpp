@@ -185,6 +185,11 @@ aclblasStatus_t EnsureWorkspace(_aclblas_handle* h, size_t bytes)
         LogError("aclblasHandle", "workspace of %zu bytes exceeds the %zu byte limit", bytes, kMaxWorkspaceSize);
         return ACLBLAS_STATUS_ALLOC_FAILED;
     }
+    if (StreamCapturing(h)) {
+        LogError("aclblasHandle", "workspace must grow to %zu bytes, which cannot happen during stream capture: "
+                 "run the call once before capturing or set a workspace", bytes);
+        return ACLBLAS_STATUS_NOT_SUPPORTED;
+    }
     aclblasStatus_t st = SyncHandleStream(h);
     if (st != ACLBLAS_STATUS_SUCCESS) {
         return st;
@@ -198,11 +203,36 @@ aclblasStatus_t EnsureWorkspace(_aclblas_handle* h, size_t bytes)
     return AllocateLibraryWorkspace(h, size);
 }
 
+bool StreamCapturing(const _aclblas_handle* h)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(h->stream, &cs) != hipSuccess) {
+        return false;
+    }
+    return cs != hipStreamCaptureStatusNone;
+}
+
 aclblasStatus_t StageToWorkspace(_aclblas_handle* h, const void* src, size_t bytes, void** dev)
 {
     aclblasStatus_t st = EnsureWorkspace(h, bytes);
     if (st != ACLBLAS_STATUS_SUCCESS) {
         return st;
+    }
+    if (StreamCapturing(h)) {
+        // A captured copy re-reads its host source at every replay: give it a
+        // pinned buffer of its own that lives as long as the handle.
+        void* keep = nullptr;
+        if (hipHostMalloc(&keep, bytes, hipHostMallocDefault) != hipSuccess) {
+            LogError("aclblasHandle", "pinned staging allocation of %zu bytes failed", bytes);
+            return ACLBLAS_STATUS_ALLOC_FAILED;
+        }
+        h->captured_stages.push_back(keep);
+        std::memcpy(keep, src, bytes);
+        if (hipMemcpyAsync(h->workspace, keep, bytes, hipMemcpyHostToDevice, h->stream) != hipSuccess) {
+            return ACLBLAS_STATUS_EXECUTION_FAILED;
+        }
+        *dev = h->workspace;
+        return ACLBLAS_STATUS_SUCCESS;
     }
     if (h->stage_event == nullptr) {
         if (hipEventCreateWithFlags(&h->stage_event, hipEventDisableTiming) != hipSuccess) {
@@ -295,6 +325,9 @@ aclblasStatus_t aclblasDestroy(aclblasHandle_t handle)
     }
     if (handle->stage_event != nullptr) {
         (void)hipEventDestroy(handle->stage_event);
+    }
+    for (void* p : handle->captured_stages) {
+        (void)hipHostFree(p);
     }
     delete handle;
     return ACLBLAS_STATUS_SUCCESS;

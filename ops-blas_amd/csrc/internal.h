@@ -8,6 +8,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
@@ -44,6 +46,9 @@ struct _aclblas_handle {
     void* host_stage = nullptr;
     size_t host_stage_size = 0;
     hipEvent_t stage_event = nullptr;
+    // Pinned copies of tables staged while the stream was being captured into
+    // a graph (each replay reads its own copy); freed with the handle.
+    std::vector<void*> captured_stages;
 };
 
 namespace aclblas {
@@ -60,6 +65,9 @@ aclblasStatus_t SyncHandleStream(_aclblas_handle* h);
 // aclblas_handle_internal.h:162-209).  Never called on the hot path: every
 // hot-path kernel here runs without a workspace.
 aclblasStatus_t EnsureWorkspace(_aclblas_handle* h, size_t bytes);
+
+// True while the handle's stream is being captured into a graph.
+bool StreamCapturing(const _aclblas_handle* h);
 
 // Copies `bytes` of host data to the start of the handle's workspace,
 // asynchronously on the handle's stream, through the handle's pinned staging

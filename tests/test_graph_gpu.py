@@ -1,0 +1,170 @@
+"""HIP graph capture of the calls that stage a per-problem table (mixed-order solve, grouped GEMM) and of a
+plain LU + solve sequence.
+
+Each call is captured once on a dedicated stream (the handle bound to it before capture, so capture sees
+no stream synchronisation and no workspace growth), then the graph is replayed several times with new
+right-hand sides / operands written between replays by ordinary stream work; every replay must equal an
+eager call on the same inputs bit for bit.  No memset is recorded: the captured work is one host-to-device
+copy of the table and the kernels.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import aclblas as B
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _pack(systems, dev):
+    offA, offB, a_parts, b_parts = [], [], [], []
+    oa = ob = 0
+    for s in systems:
+        offA.append(oa)
+        offB.append(ob)
+        a_parts.append(s["A"])
+        b_parts.append(s["B"])
+        oa += s["A"].size
+        ob += s["B"].size
+    return (torch.from_numpy(np.concatenate(a_parts)).to(dev), torch.from_numpy(np.concatenate(b_parts)).to(dev),
+            offA, offB)
+
+
+def test_mixed_solve_graph_replays():
+    dev = torch.device("cuda")
+    stream = torch.cuda.Stream()
+    h = B.Handle()
+    assert h.set_stream(stream.cuda_stream) == 0
+    try:
+        systems = W.conditioned_systems(48, seed=21)
+        dA, dB, offA, offB = _pack(systems, dev)
+        dX = torch.zeros_like(dB)
+        ns = [s["n"] for s in systems]
+        rs = [s["nrhs"] for s in systems]
+        pa = [dA.data_ptr() + 4 * o for o in offA]
+        pb = [dB.data_ptr() + 4 * o for o in offB]
+        px = [dX.data_ptr() + 4 * o for o in offB]
+        iters = torch.zeros(len(systems), dtype=torch.int32, device=dev)
+
+        def solve(xptrs, it):
+            return B.sgesv_batched_mixed(h.value, ns, rs, pa, ns, pb, ns, xptrs, ns, 1e-9, 8, it.data_ptr(), None,
+                                         None)
+
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            assert solve(px, iters) == 0          # eager first call: sizes the workspace
+        stream.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream, capture_error_mode="relaxed"):
+            assert solve(px, iters) == 0
+        dX2 = torch.zeros_like(dB)
+        px2 = [dX2.data_ptr() + 4 * o for o in offB]
+        it2 = torch.zeros_like(iters)
+        rng = np.random.default_rng(5)
+        for rep in range(4):
+            newB = torch.from_numpy(rng.standard_normal(dB.numel()).astype(np.float32)).to(dev)
+            with torch.cuda.stream(stream):
+                dB.copy_(newB)
+                dX.zero_()
+                g.replay()
+                assert solve(px2, it2) == 0       # eager reference on the same inputs
+            stream.synchronize()
+            assert torch.equal(dX, dX2), "replay %d differs from the eager call" % rep
+            assert torch.equal(iters, it2)
+    finally:
+        torch.cuda.synchronize()
+        h.destroy()
+
+
+def test_grouped_gemm_graph_replays():
+    dev = torch.device("cuda")
+    stream = torch.cuda.Stream()
+    h = B.Handle()
+    assert h.set_stream(stream.cuda_stream) == 0
+    try:
+        shapes = [(300, 129, 77), (64, 64, 512), (129, 257, 33)]
+        As = [torch.randn(m * k, device=dev) for m, n, k in shapes]
+        Bs = [torch.randn(k * n, device=dev) for m, n, k in shapes]
+        Cs = [torch.randn(m * n, device=dev) for m, n, k in shapes]
+        C0 = [c.clone() for c in Cs]
+        ms, nn, ks = [s[0] for s in shapes], [s[1] for s in shapes], [s[2] for s in shapes]
+
+        def call(cs):
+            return B.sgemm_grouped_batched(h.value, [B.OP_N] * 3, [B.OP_N] * 3, ms, nn, ks, [1.5] * 3,
+                                           [a.data_ptr() for a in As], ms, [b.data_ptr() for b in Bs], ks,
+                                           [0.5] * 3, [c.data_ptr() for c in cs], ms, [1] * 3)
+
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            assert call(Cs) == 0
+        stream.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream, capture_error_mode="relaxed"):
+            assert call(Cs) == 0
+        Ref = [torch.empty_like(c) for c in Cs]
+        for rep in range(3):
+            with torch.cuda.stream(stream):
+                for a in As:
+                    a.normal_()
+                for c, c0, r in zip(Cs, C0, Ref):
+                    c.copy_(c0)
+                    r.copy_(c0)
+                g.replay()
+                assert call(Ref) == 0
+            stream.synchronize()
+            for c, r in zip(Cs, Ref):
+                assert torch.equal(c, r), "replay %d differs" % rep
+    finally:
+        torch.cuda.synchronize()
+        h.destroy()
+
+
+def test_lu_solve_sequence_graph_replays():
+    """aclblasSgetrfBatched + aclblasSgetrsBatched captured as one factor + solve sequence."""
+    dev = torch.device("cuda")
+    stream = torch.cuda.Stream()
+    h = B.Handle()
+    assert h.set_stream(stream.cuda_stream) == 0
+    try:
+        batch, n = 512, 32
+        A0, B0 = W.lu_batch(batch, n, 2, seed=3)
+        dA0 = torch.from_numpy(A0).to(dev)
+        dB0 = torch.from_numpy(B0).to(dev)
+        LU, X = dA0.clone(), dB0.clone()
+        piv = torch.zeros((batch, n), dtype=torch.int32, device=dev)
+        info = torch.zeros(batch, dtype=torch.int32, device=dev)
+        pLU, pX = B.ptr_array(LU), B.ptr_array(X)
+        hinfo = ctypes.c_int(0)
+
+        def seq():
+            st = B.sgetrf_batched(h.value, n, pLU.data_ptr(), n, piv.data_ptr(), info.data_ptr(), batch)
+            st |= B.sgetrs_batched(h.value, B.OP_N, n, 2, pLU.data_ptr(), n, piv.data_ptr(), pX.data_ptr(), n,
+                                   hinfo, batch)
+            return st
+
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream):
+            assert seq() == 0
+        LU2, X2 = dA0.clone(), dB0.clone()
+        piv2 = torch.zeros_like(piv)
+        pLU2, pX2 = B.ptr_array(LU2), B.ptr_array(X2)
+        for rep in range(3):
+            with torch.cuda.stream(stream):
+                LU.copy_(dA0 + rep)
+                X.copy_(dB0 * (rep + 1))
+                LU2.copy_(dA0 + rep)
+                X2.copy_(dB0 * (rep + 1))
+                g.replay()
+                st = B.sgetrf_batched(h.value, n, pLU2.data_ptr(), n, piv2.data_ptr(), info.data_ptr(), batch)
+                st |= B.sgetrs_batched(h.value, B.OP_N, n, 2, pLU2.data_ptr(), n, piv2.data_ptr(), pX2.data_ptr(),
+                                       n, hinfo, batch)
+                assert st == 0
+            stream.synchronize()
+            assert torch.equal(LU, LU2) and torch.equal(piv, piv2) and torch.equal(X, X2), rep
+    finally:
+        torch.cuda.synchronize()
+        h.destroy()
