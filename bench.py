@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--primary-only", action="store_true", help="skip the secondary workloads")
     ap.add_argument("--profile-gemm", action="store_true",
                     help="only run three BF16 x BF16 -> FP32 8192^3 GEMMs (for counter collection) and exit")
+    ap.add_argument("--profile-mixed", action="store_true",
+                    help="only run the mixed-order solve of 10000 systems three times (for profiling) and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="before warm-up, run 5 elementwise negations of a MIB-MiB FP32 tensor (read and write "
                          "MIB MiB each): the known-size reference that scales hardware byte counters")
@@ -111,7 +113,7 @@ class Timer:
         return e0.elapsed_time(e1) / reps
 
 
-def bench_mixed(torch, B, W, h, timer):
+def bench_mixed(torch, B, W, h, timer, plain=True):
     """10000 mixed-order systems (n 16..256, nrhs 1..16, cond 1e2..1e6): BF16 matrix-core LU + FP64
     refinement in two persistent launches, against the plain FP32 path (aclblasSgetrfBatched +
     aclblasSgetrsBatched per order, right-hand sides padded to the order's largest nrhs)."""
@@ -154,6 +156,8 @@ def bench_mixed(torch, B, W, h, timer):
     be_mixed = max(W.normwise_backward_error(s["A"], s["n"], s["B"], X[o:o + s["n"] * s["nrhs"]], s["nrhs"])
                    for s, o in zip(systems, offB))
 
+    if not plain:
+        return {"systems": count, "mixed_ms": mixed_ms, "mixed_max_backward_error": be_mixed}
     # plain FP32 path: per order, copies of A (getrf is in place) and padded right-hand sides
     groups = {}
     for i, s in enumerate(systems):
@@ -304,6 +308,14 @@ def main():
     B.load()
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.profile_mixed:
+        h = B.Handle()
+        stream = torch.cuda.current_stream()
+        h.set_stream(stream.cuda_stream)
+        res = bench_mixed(torch, B, W, h, Timer(torch, stream), plain=False)
+        h.destroy()
+        print(json.dumps(res))
+        return
     if args.profile_gemm:
         h = B.Handle()
         h.set_stream(torch.cuda.current_stream().cuda_stream)

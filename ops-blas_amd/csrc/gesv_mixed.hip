@@ -360,9 +360,19 @@ __global__ __launch_bounds__(kThreads, 2) void GesvFactorKernel(const MixedSyste
         const MixedSystem s = systems[idx];
         const int n = s.n;
         // working copy of A
-        for (int j = 0; j < n; ++j) {
-            for (int i = t; i < n; i += kThreads) {
-                s.LU[i + static_cast<size_t>(j) * n] = s.A[i + static_cast<size_t>(j) * s.lda];
+        if (t < n) {
+            for (int j0 = 0; j0 < n; j0 += 8) {
+                float v[8];
+                dev::StaticFor<0, 8>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    v[j] = j0 + j < n ? s.A[t + static_cast<size_t>(j0 + j) * s.lda] : 0.0f;
+                });
+                dev::StaticFor<0, 8>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    if (j0 + j < n) {
+                        s.LU[t + static_cast<size_t>(j0 + j) * n] = v[j];
+                    }
+                });
             }
         }
         if (t == 0) {
@@ -395,7 +405,6 @@ __global__ __launch_bounds__(kThreads, 2) void GesvFactorKernel(const MixedSyste
 // ---------------------------------------------------------------------------
 struct RefineShared {
     float bs[kMaxN][kMaxRhs];
-    float tmp[kMaxN][kMaxRhs];
     double x[kMaxN][kMaxRhs];
     double red[3][kWaves][kMaxRhs];  // per-wave max |r|, |x|, |b|
     double anorm[kWaves];            // per-wave max row sum of |A|
@@ -424,50 +433,57 @@ __device__ __forceinline__ void FactoredSolve(const MixedSystem& s, RefineShared
         const int rows = n - k0;
         const int w = min(kNb, rows);
         const int* map = s.rowmap + RowMapOffset(n, p);
-        if (t < rows) {
-            const int src = map[t];
-            for (int c = 0; c < nrhs; ++c) {
-                sh.tmp[t][c] = sh.bs[k0 + src][c];
-            }
+        {
+            // gather through the row map in registers, then write back in place
+            float g[kMaxRhs];
+            const int src = t < rows ? map[t] : 0;
+            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
+                constexpr int c = decltype(cc)::value;
+                g[c] = (t < rows && c < nrhs) ? sh.bs[k0 + src][c] : 0.0f;
+            });
+            __syncthreads();
+            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
+                constexpr int c = decltype(cc)::value;
+                if (t < rows && c < nrhs) {
+                    sh.bs[k0 + t][c] = g[c];
+                }
+            });
         }
         __syncthreads();
-        if (t < rows) {
-            for (int c = 0; c < nrhs; ++c) {
-                sh.bs[k0 + t][c] = sh.tmp[t][c];
-            }
-        }
-        __syncthreads();
-        if (wave == 0) {
+        {
+            // diagonal block: lane q holds row q; wave v solves right-hand sides c = v, v+4, ...
             const bool valid = lane < w;
             float l[kNb];
-            float x[kMaxRhs];
+            float x[kMaxRhs / kWaves];
             dev::StaticFor<0, kNb>([&](auto jc) {
                 constexpr int j = decltype(jc)::value;
                 l[j] = (valid && j < lane) ? LU[(k0 + lane) + static_cast<size_t>(k0 + j) * n] : 0.0f;
             });
-            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                x[c] = (valid && c < nrhs) ? sh.bs[k0 + lane][c] : 0.0f;
+            dev::StaticFor<0, kMaxRhs / kWaves>([&](auto cc) {
+                constexpr int ci = decltype(cc)::value;
+                const int c = ci * kWaves + wave;
+                x[ci] = (valid && c < nrhs) ? sh.bs[k0 + lane][c] : 0.0f;
             });
             dev::StaticFor<0, kNb>([&](auto jc) {
                 constexpr int j = decltype(jc)::value;
                 if (j < w) {
-                    dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                        constexpr int c = decltype(cc)::value;
-                        if (c < nrhs) {
-                            const float xj = dev::ReadLane(x[c], j);
+                    dev::StaticFor<0, kMaxRhs / kWaves>([&](auto cc) {
+                        constexpr int ci = decltype(cc)::value;
+                        if (ci * kWaves + wave < nrhs) {
+                            const float xj = dev::ReadLane(x[ci], j);
                             if (lane > j) {
-                                x[c] = fmaf(-l[j], xj, x[c]);
+                                x[ci] = fmaf(-l[j], xj, x[ci]);
                             }
                         }
                     });
                 }
             });
             if (valid) {
-                dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                    constexpr int c = decltype(cc)::value;
+                dev::StaticFor<0, kMaxRhs / kWaves>([&](auto cc) {
+                    constexpr int ci = decltype(cc)::value;
+                    const int c = ci * kWaves + wave;
                     if (c < nrhs) {
-                        sh.bs[k0 + lane][c] = x[c];
+                        sh.bs[k0 + lane][c] = x[ci];
                     }
                 });
             }
@@ -480,15 +496,22 @@ __device__ __forceinline__ void FactoredSolve(const MixedSystem& s, RefineShared
                 constexpr int c = decltype(cc)::value;
                 acc[c] = c < nrhs ? sh.bs[i][c] : 0.0f;
             });
-            for (int j = 0; j < w; ++j) {
-                const float lv = LU[i + static_cast<size_t>(k0 + j) * n];
-                dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                    constexpr int c = decltype(cc)::value;
-                    if (c < nrhs) {
-                        acc[c] = fmaf(-lv, sh.bs[k0 + j][c], acc[c]);
-                    }
-                });
-            }
+            float lv[kNb];
+            dev::StaticFor<0, kNb>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                lv[j] = j < w ? LU[i + static_cast<size_t>(k0 + j) * n] : 0.0f;
+            });
+            dev::StaticFor<0, kNb>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                if (j < w) {
+                    dev::StaticFor<0, kMaxRhs>([&](auto cc) {
+                        constexpr int c = decltype(cc)::value;
+                        if (c < nrhs) {
+                            acc[c] = fmaf(-lv[j], sh.bs[k0 + j][c], acc[c]);
+                        }
+                    });
+                }
+            });
             dev::StaticFor<0, kMaxRhs>([&](auto cc) {
                 constexpr int c = decltype(cc)::value;
                 if (c < nrhs) {
@@ -503,40 +526,42 @@ __device__ __forceinline__ void FactoredSolve(const MixedSystem& s, RefineShared
     for (int bi = blocks - 1; bi >= 0; --bi) {
         const int r0 = bi * kNb;
         const int w = min(kNb, n - r0);
-        if (wave == 0) {
+        {
             const bool valid = lane < w;
             float u[kNb];
-            float x[kMaxRhs];
+            float x[kMaxRhs / kWaves];
             dev::StaticFor<0, kNb>([&](auto jc) {
                 constexpr int j = decltype(jc)::value;
                 u[j] = (valid && j >= lane && j < w) ? LU[(r0 + lane) + static_cast<size_t>(r0 + j) * n] : 0.0f;
             });
-            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                x[c] = (valid && c < nrhs) ? sh.bs[r0 + lane][c] : 0.0f;
+            dev::StaticFor<0, kMaxRhs / kWaves>([&](auto cc) {
+                constexpr int ci = decltype(cc)::value;
+                const int c = ci * kWaves + wave;
+                x[ci] = (valid && c < nrhs) ? sh.bs[r0 + lane][c] : 0.0f;
             });
             dev::StaticFor<0, kNb>([&](auto jc) {
                 constexpr int j = kNb - 1 - decltype(jc)::value;
                 if (j < w) {
-                    dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                        constexpr int c = decltype(cc)::value;
-                        if (c < nrhs) {
+                    dev::StaticFor<0, kMaxRhs / kWaves>([&](auto cc) {
+                        constexpr int ci = decltype(cc)::value;
+                        if (ci * kWaves + wave < nrhs) {
                             if (lane == j) {
-                                x[c] = x[c] / u[j];
+                                x[ci] = x[ci] / u[j];
                             }
-                            const float xj = dev::ReadLane(x[c], j);
+                            const float xj = dev::ReadLane(x[ci], j);
                             if (lane < j) {
-                                x[c] = fmaf(-u[j], xj, x[c]);
+                                x[ci] = fmaf(-u[j], xj, x[ci]);
                             }
                         }
                     });
                 }
             });
             if (valid) {
-                dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                    constexpr int c = decltype(cc)::value;
+                dev::StaticFor<0, kMaxRhs / kWaves>([&](auto cc) {
+                    constexpr int ci = decltype(cc)::value;
+                    const int c = ci * kWaves + wave;
                     if (c < nrhs) {
-                        sh.bs[r0 + lane][c] = x[c];
+                        sh.bs[r0 + lane][c] = x[ci];
                     }
                 });
             }
@@ -548,15 +573,22 @@ __device__ __forceinline__ void FactoredSolve(const MixedSystem& s, RefineShared
                 constexpr int c = decltype(cc)::value;
                 acc[c] = c < nrhs ? sh.bs[t][c] : 0.0f;
             });
-            for (int j = 0; j < w; ++j) {
-                const float uv = LU[t + static_cast<size_t>(r0 + j) * n];
-                dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                    constexpr int c = decltype(cc)::value;
-                    if (c < nrhs) {
-                        acc[c] = fmaf(-uv, sh.bs[r0 + j][c], acc[c]);
-                    }
-                });
-            }
+            float uv[kNb];
+            dev::StaticFor<0, kNb>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                uv[j] = j < w ? LU[t + static_cast<size_t>(r0 + j) * n] : 0.0f;
+            });
+            dev::StaticFor<0, kNb>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                if (j < w) {
+                    dev::StaticFor<0, kMaxRhs>([&](auto cc) {
+                        constexpr int c = decltype(cc)::value;
+                        if (c < nrhs) {
+                            acc[c] = fmaf(-uv[j], sh.bs[r0 + j][c], acc[c]);
+                        }
+                    });
+                }
+            });
             dev::StaticFor<0, kMaxRhs>([&](auto cc) {
                 constexpr int c = decltype(cc)::value;
                 if (c < nrhs) {
@@ -568,7 +600,7 @@ __device__ __forceinline__ void FactoredSolve(const MixedSystem& s, RefineShared
     }
 }
 
-__global__ __launch_bounds__(kThreads, 2) void GesvRefineKernel(const MixedSystem* __restrict__ systems, int count,
+__global__ __launch_bounds__(kThreads, 3) void GesvRefineKernel(const MixedSystem* __restrict__ systems, int count,
                                                             double tol, int maxIter, int* __restrict__ iterOut,
                                                             float* __restrict__ berrOut, int* __restrict__ infoOut)
 {
@@ -618,13 +650,23 @@ __global__ __launch_bounds__(kThreads, 2) void GesvRefineKernel(const MixedSyste
                 r[c] = (t < n && c < nrhs) ? static_cast<double>(s.B[t + static_cast<size_t>(c) * s.ldb]) : 0.0;
             });
             if (t < n) {
-                for (int j = 0; j < n; ++j) {
-                    const double av = static_cast<double>(s.A[t + static_cast<size_t>(j) * s.lda]);
-                    rs += fabs(av);
-                    dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                        constexpr int c = decltype(cc)::value;
-                        if (c < nrhs) {
-                            r[c] = fma(-av, sh.x[j][c], r[c]);
+                for (int j0 = 0; j0 < n; j0 += 8) {
+                    float av[8];
+                    dev::StaticFor<0, 8>([&](auto jc) {
+                        constexpr int j = decltype(jc)::value;
+                        av[j] = j0 + j < n ? s.A[t + static_cast<size_t>(j0 + j) * s.lda] : 0.0f;
+                    });
+                    dev::StaticFor<0, 8>([&](auto jc) {
+                        constexpr int j = decltype(jc)::value;
+                        if (j0 + j < n) {
+                            const double a = static_cast<double>(av[j]);
+                            rs += fabs(a);
+                            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
+                                constexpr int c = decltype(cc)::value;
+                                if (c < nrhs) {
+                                    r[c] = fma(-a, sh.x[j0 + j][c], r[c]);
+                                }
+                            });
                         }
                     });
                 }
@@ -826,10 +868,12 @@ extern "C" aclblasStatus_t aclblasSgesvBatchedMixed(
         return st;
     }
     const int cus = handle->cu_count > 0 ? handle->cu_count : 256;
-    const unsigned grid = static_cast<unsigned>(std::min(batchCount, 2 * cus));
+    // persistent grids: two factor workgroups and three solve workgroups per CU fit (LDS and registers)
+    const unsigned gridF = static_cast<unsigned>(std::min(batchCount, 2 * cus));
+    const unsigned gridS = static_cast<unsigned>(std::min(batchCount, 3 * cus));
     const MixedSystem* sys = static_cast<const MixedSystem*>(dev);
-    hipLaunchKernelGGL(GesvFactorKernel, dim3(grid), dim3(kThreads), 0, handle->stream, sys, batchCount);
-    hipLaunchKernelGGL(GesvRefineKernel, dim3(grid), dim3(kThreads), 0, handle->stream, sys, batchCount, tol, maxIter,
-                       iterArray, berrArray, infoArray);
+    hipLaunchKernelGGL(GesvFactorKernel, dim3(gridF), dim3(kThreads), 0, handle->stream, sys, batchCount);
+    hipLaunchKernelGGL(GesvRefineKernel, dim3(gridS), dim3(kThreads), 0, handle->stream, sys, batchCount, tol,
+                       maxIter, iterArray, berrArray, infoArray);
     return LaunchStatus("aclblasSgesvBatchedMixed");
 }
