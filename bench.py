@@ -145,8 +145,13 @@ def bench_mixed(torch, B, W, h, timer, plain=True):
     berr = torch.zeros(count, dtype=torch.float32, device=dev)
     info = torch.zeros(count, dtype=torch.int32, device=dev)
 
+    # host arrays built once, as a C caller holds them (the timed call still sorts the systems, builds and
+    # stages the per-system table)
+    hns, hrs = B.host_ints(ns), B.host_ints(rs)
+    hpa, hpb, hpx = B.host_ptrs(pa), B.host_ptrs(pb), B.host_ptrs(px)
+
     def mixed():
-        st = B.sgesv_batched_mixed(h.value, ns, rs, pa, ns, pb, ns, px, ns, 1e-6, 10, iters.data_ptr(),
+        st = B.sgesv_batched_mixed(h.value, hns, hrs, hpa, hns, hpb, hns, hpx, hns, 1e-6, 10, iters.data_ptr(),
                                    berr.data_ptr(), info.data_ptr())
         assert st == 0, st
 

@@ -225,6 +225,34 @@ aclblasStatus_t aclblasSgesvBatchedMixed(
     const int* ldaArray, const float* const Barray[], const int* ldbArray, float* const Xarray[],
     const int* ldxArray, double tol, int maxIter, int* iterArray, float* berrArray, int* infoArray);
 
+/* ---- launch tuning (no reference counterpart: the reference picks its tiles
+ * with fixed host rules, e.g. blas/gemm_grouped_batched/arch35/
+ * gemm_grouped_batched_host.cpp:98-124).  Calls with several launch shapes map
+ * their problem to a size bucket and take the shape from a table measured by
+ * ops-blas_amd/autotune.py.  Operations and their two parameters:
+ *   0 GEMM (GemmEx, Sgemm, SgemmEx, GemmBatchedEx, GemmExScaled):
+ *     a = output tile edge 64 / 128 / 256 (256: 1- and 2-byte operands only),
+ *     b = tile rows walked together per L2 group, 1..64;
+ *   1 grouped GEMM: a = tile edge 64 / 128, b = persistent workgroups per CU 1..4;
+ *   2 mixed solve: a = factor workgroups per CU 1..4, b = solve workgroups per CU 1..4.
+ * Results do not depend on the choice (every output element sums its products
+ * in the same order under every tile shape and grid). */
+
+/* Forces (a, b) for every bucket of `op` on this handle; a < 0 removes the
+ * override.  INVALID_VALUE when (a, b) is legal for no bucket of `op`; a forced
+ * choice that a bucket cannot take (a 256 tile for FP32) leaves that bucket on
+ * the table. */
+aclblasStatus_t aclblasSetTuningOverride(aclblasHandle_t handle, int op, int a, int b);
+
+/* The choice the handle uses for (op, bucket): override, table or built-in. */
+aclblasStatus_t aclblasGetTuningChoice(aclblasHandle_t handle, int op, int bucket, int* a, int* b);
+
+/* Bucket of a problem: op 0 takes cls (0 FP32, 1 BF16/FP16, 2 FP8 under
+ * GemmEx, 3 FP8 under GemmExScaled), units = 128 x 128 output tiles over the
+ * call and k; op 1 takes units = 128 x 128 tiles over all problems; op 2 takes
+ * units = number of systems.  cls and k are ignored for ops 1 and 2. */
+aclblasStatus_t aclblasTuningBucket(aclblasHandle_t handle, int op, int cls, long long units, int k, int* bucket);
+
 #ifdef __cplusplus
 }
 #endif

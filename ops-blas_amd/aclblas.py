@@ -65,7 +65,15 @@ _SIGNATURES = {
     "aclblasSgesvBatchedMixed": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, _I, _P, _P, _P],
     "aclblasGemmExScaled": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _P, _P, _I, _I, _P, _P, _P, _I, _I],
     "aclblasQuantizeFp8": [_P, ctypes.c_longlong, _P, _I, _P, _P],
+    "aclblasSetTuningOverride": [_P, _I, _I, _I],
+    "aclblasGetTuningChoice": [_P, _I, _I, _P, _P],
+    "aclblasTuningBucket": [_P, _I, _I, ctypes.c_longlong, _I, _P],
 }
+
+# tuning operations (aclblasSetTuningOverride)
+TUNE_GEMM, TUNE_GROUPED, TUNE_MIXED = 0, 1, 2
+# GEMM bucket element classes
+CLASS_F32, CLASS_16, CLASS_F8, CLASS_F8_SCALED = 0, 1, 2, 3
 
 
 def load(path=None):
@@ -166,10 +174,21 @@ def sgetri_batched(h, n, Aarray, lda, piv, Carray, ldc, info, batch):
 
 
 def _host_array(ctype, values):
-    if values is None:
-        return None
+    """Python list -> ctypes host array; a ctypes array (host_ints / host_ptrs) passes through."""
+    if values is None or isinstance(values, ctypes.Array):
+        return values
     arr = (ctype * max(1, len(values)))(*values)
     return arr
+
+
+def host_ints(values):
+    """Host int array for the calls that take host arrays (built once, reusable across calls)."""
+    return (ctypes.c_int * max(1, len(values)))(*values)
+
+
+def host_ptrs(values):
+    """Host array of device addresses for the calls that take host pointer arrays."""
+    return (ctypes.c_void_p * max(1, len(values)))(*values)
 
 
 def sgemm_grouped_batched(h, transa, transb, m, n, k, alpha, A, lda, Bp, ldb, beta, C, ldc, group_size):
@@ -212,6 +231,25 @@ def sgetrs_batched_adjoint(h, n, nrhs, LUarray, lda, piv, Xarray, ldx, dXarray, 
     """dX <- A^-T dX (in place, the gradient of B) and dA <- -(A^-T dX) X^T; dAarray may be None."""
     return load().aclblasSgetrsBatchedAdjoint(h, n, nrhs, LUarray, lda, piv, Xarray, ldx, dXarray, lddx, dAarray,
                                               ldda, batch)
+
+
+def set_tuning_override(h, op, a, b=0):
+    """Forces launch choice (a, b) for every bucket of `op` on handle h; a < 0 removes it."""
+    return load().aclblasSetTuningOverride(h, op, a, b)
+
+
+def get_tuning_choice(h, op, bucket):
+    """(status, a, b) of the choice handle h uses for (op, bucket)."""
+    a, b = ctypes.c_int(0), ctypes.c_int(0)
+    st = load().aclblasGetTuningChoice(h, op, bucket, ctypes.byref(a), ctypes.byref(b))
+    return st, a.value, b.value
+
+
+def tuning_bucket(h, op, cls, units, k=0):
+    """(status, bucket) of a problem (see aclblasTuningBucket)."""
+    out = ctypes.c_int(-1)
+    st = load().aclblasTuningBucket(h, op, cls, units, k, ctypes.byref(out))
+    return st, out.value
 
 
 # ---------------------------------------------------------------- torch helpers

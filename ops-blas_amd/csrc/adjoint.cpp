@@ -72,7 +72,7 @@ extern "C" aclblasStatus_t aclblasSgetrsBatchedAdjoint(
         return ACLBLAS_STATUS_SUCCESS;
     }
     // dA = -dB X^T  (k = nrhs; nrhs == 0 gives dA = 0)
-    return SgemmBatchedLaunch(handle->stream, handle->cu_count, false, true, n, n, nrhs, -1.0f,
+    return SgemmBatchedLaunch(handle, false, true, n, n, nrhs, -1.0f,
                               const_cast<const float* const*>(dXarray), lddx, Xarray, ldx, 0.0f, dAarray, ldda,
                               batchCount);
 }

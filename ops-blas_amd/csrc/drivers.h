@@ -6,6 +6,8 @@
 
 #include "cann_ops_blas.h"
 
+struct _aclblas_handle;
+
 namespace aclblas {
 
 // LU with partial pivoting (piv == nullptr: no pivoting), getrf.hip.
@@ -15,8 +17,9 @@ aclblasStatus_t GetrfBatchedLaunch(hipStream_t st, int n, float* const* A, int l
 aclblasStatus_t GetrsBatchedLaunch(hipStream_t st, bool trans, int n, int nrhs, const float* const* A, int lda,
                                    const int* piv, float* const* B, int ldb, int batch);
 
-// C[i] = alpha op(A[i]) op(B[i]) + beta C[i], FP32, device pointer arrays, gemm.hip.
-aclblasStatus_t SgemmBatchedLaunch(hipStream_t st, int cus, bool transA, bool transB, int m, int n, int k,
+// C[i] = alpha op(A[i]) op(B[i]) + beta C[i], FP32, device pointer arrays, on the
+// handle's stream with its tuned tile choice, gemm.hip.
+aclblasStatus_t SgemmBatchedLaunch(const _aclblas_handle* h, bool transA, bool transB, int m, int n, int k,
                                    float alpha, const float* const* A, int lda, const float* const* B, int ldb,
                                    float beta, float* const* C, int ldc, int batch);
 

@@ -13,6 +13,7 @@
 #include "gemm_kernel.h"
 #include "drivers.h"
 #include "internal.h"
+#include "tuning.h"
 
 namespace aclblas {
 namespace {
@@ -124,7 +125,7 @@ int ElemBytes(int kind) { return kind == kF32 ? 4 : (kind == kBF16 || kind == kF
 // Launches the fused GEMM for a validated, non-degenerate problem (one
 // problem when Aarr is null, otherwise `batch` problems through device
 // pointer arrays).
-aclblasStatus_t RunGemm(hipStream_t st, int cus, int ka, int kb, int ok, bool transA, bool transB, int m, int n,
+aclblasStatus_t RunGemm(const _aclblas_handle* h, int ka, int kb, int ok, bool transA, bool transB, int m, int n,
                         int k, float alpha, const void* A, int lda, const void* B, int ldb, float beta, void* C,
                         int ldc, const float* scaleA, const float* scaleB, const void* const* Aarr = nullptr,
                         const void* const* Barr = nullptr, void* const* Carr = nullptr, int batch = 1)
@@ -159,24 +160,33 @@ aclblasStatus_t RunGemm(hipStream_t st, int cus, int ka, int kb, int ok, bool tr
         p.vecC = (reinterpret_cast<uintptr_t>(c0) % (4 * cb) == 0 && (static_cast<long long>(ldc) * cb) % (4 * cb) == 0)
                      ? 1 : 0;
     }
+    const hipStream_t st = h->stream;
+    const int cus = h->cu_count > 0 ? h->cu_count : 256;
+    const int eb = ElemBytes(ka);
+    const int cls = eb == 4 ? tune::kClassF32 : (eb == 2 ? tune::kClass16 : tune::kClassF8);
+    const long long t128 = static_cast<long long>((m + 127) / 128) * ((n + 127) / 128) * batch;
+    const tune::Choice c = tune::Get(h, tune::kOpGemm, tune::GemmBucket(cls, t128, k, cus),
+                                     tune::Choice{DefaultGemmTile(eb, m, n, batch, cus), 8});
+    const int tile = c.a;
+    const int group = c.b;
     if (ka == kF32 && kb == kF32 && ok == kOutF32) {
-        LaunchGemmTyped<kF32, kF32, kOutF32>(st, p, batch, cus);
+        LaunchGemmTyped<kF32, kF32, kOutF32>(st, p, batch, tile, group);
     } else if (ka == kBF16 && kb == kBF16 && ok == kOutBF16) {
-        LaunchGemmTyped<kBF16, kBF16, kOutBF16>(st, p, batch, cus);
+        LaunchGemmTyped<kBF16, kBF16, kOutBF16>(st, p, batch, tile, group);
     } else if (ka == kBF16 && kb == kBF16 && ok == kOutF32) {
-        LaunchGemmTyped<kBF16, kBF16, kOutF32>(st, p, batch, cus);
+        LaunchGemmTyped<kBF16, kBF16, kOutF32>(st, p, batch, tile, group);
     } else if (ka == kF16 && kb == kF16 && ok == kOutF16) {
-        LaunchGemmTyped<kF16, kF16, kOutF16>(st, p, batch, cus);
+        LaunchGemmTyped<kF16, kF16, kOutF16>(st, p, batch, tile, group);
     } else if (ka == kF16 && kb == kF16 && ok == kOutF32) {
-        LaunchGemmTyped<kF16, kF16, kOutF32>(st, p, batch, cus);
+        LaunchGemmTyped<kF16, kF16, kOutF32>(st, p, batch, tile, group);
     } else if (ka == kE4M3 && kb == kE4M3) {
-        LaunchGemmTyped<kE4M3, kE4M3, kOutF16>(st, p, batch, cus);
+        LaunchGemmTyped<kE4M3, kE4M3, kOutF16>(st, p, batch, tile, group);
     } else if (ka == kE4M3 && kb == kE5M2) {
-        LaunchGemmTyped<kE4M3, kE5M2, kOutF16>(st, p, batch, cus);
+        LaunchGemmTyped<kE4M3, kE5M2, kOutF16>(st, p, batch, tile, group);
     } else if (ka == kE5M2 && kb == kE4M3) {
-        LaunchGemmTyped<kE5M2, kE4M3, kOutF16>(st, p, batch, cus);
+        LaunchGemmTyped<kE5M2, kE4M3, kOutF16>(st, p, batch, tile, group);
     } else if (ka == kE5M2 && kb == kE5M2) {
-        LaunchGemmTyped<kE5M2, kE5M2, kOutF16>(st, p, batch, cus);
+        LaunchGemmTyped<kE5M2, kE5M2, kOutF16>(st, p, batch, tile, group);
     } else {
         return ACLBLAS_STATUS_NOT_SUPPORTED;
     }
@@ -278,7 +288,7 @@ aclblasStatus_t ValidateGemmEx(aclblasHandle_t handle, aclblasOperation_t ta, ac
 
 // Internal entry shared with the solve drivers: validated FP32/typed GEMM on a
 // stream, alpha/beta by value.
-aclblasStatus_t GemmLaunch(hipStream_t st, int cus, aclDataType at, aclDataType bt, aclDataType ct, bool transA,
+aclblasStatus_t GemmLaunch(const _aclblas_handle* h, aclDataType at, aclDataType bt, aclDataType ct, bool transA,
                            bool transB, int m, int n, int k, float alpha, const void* A, int lda, const void* B,
                            int ldb, float beta, void* C, int ldc, const float* scaleA, const float* scaleB)
 {
@@ -288,12 +298,12 @@ aclblasStatus_t GemmLaunch(hipStream_t st, int cus, aclDataType at, aclDataType 
     const int ok = OutKindOf(ct);
     if (k == 0 || alpha == 0.0f) {
         if (beta != 1.0f) {
-            LaunchScale(st, ok, C, m, n, ldc, beta);
+            LaunchScale(h->stream, ok, C, m, n, ldc, beta);
             return LaunchStatus("aclblasGemm");
         }
         return ACLBLAS_STATUS_SUCCESS;
     }
-    return RunGemm(st, cus, KindOf(at), KindOf(bt), ok, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C,
+    return RunGemm(h, KindOf(at), KindOf(bt), ok, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C,
                    ldc, scaleA, scaleB);
 }
 
@@ -327,7 +337,7 @@ extern "C" aclblasStatus_t aclblasSgemmEx(
     if (!HandleHasDevice(handle)) {
         return ACLBLAS_STATUS_EXECUTION_FAILED;
     }
-    return GemmLaunch(handle->stream, handle->cu_count, ACL_FLOAT, ACL_FLOAT, ACL_FLOAT, transA != ACLBLAS_OP_N,
+    return GemmLaunch(handle, ACL_FLOAT, ACL_FLOAT, ACL_FLOAT, transA != ACLBLAS_OP_N,
                       transB != ACLBLAS_OP_N, m, n, k, a, A, lda, B, ldb, b, C, ldc, nullptr, nullptr);
 }
 
@@ -361,7 +371,7 @@ extern "C" aclblasStatus_t aclblasGemmEx(
     if (!HandleHasDevice(handle)) {
         return ACLBLAS_STATUS_EXECUTION_FAILED;
     }
-    return GemmLaunch(handle->stream, handle->cu_count, Atype, Btype, Ctype, transA != ACLBLAS_OP_N,
+    return GemmLaunch(handle, Atype, Btype, Ctype, transA != ACLBLAS_OP_N,
                       transB != ACLBLAS_OP_N, m, n, k,
                       a, A, lda, B, ldb, b, C, ldc, nullptr, nullptr);
 }
@@ -463,13 +473,13 @@ aclblasStatus_t RunBatched(aclblasHandle_t handle, aclDataType at, aclDataType b
         LaunchScaleBatched(handle->stream, ok, C, m, n, ldc, beta, batch);
         return LaunchStatus("aclblasGemmBatched");
     }
-    return RunGemm(handle->stream, handle->cu_count, KindOf(at), KindOf(bt), ok, tA, tB, m, n, k, alpha, nullptr,
+    return RunGemm(handle, KindOf(at), KindOf(bt), ok, tA, tB, m, n, k, alpha, nullptr,
                    lda, nullptr, ldb, beta, nullptr, ldc, nullptr, nullptr, A, B, C, batch);
 }
 
 }  // namespace
 
-aclblasStatus_t SgemmBatchedLaunch(hipStream_t st, int cus, bool transA, bool transB, int m, int n, int k,
+aclblasStatus_t SgemmBatchedLaunch(const _aclblas_handle* h, bool transA, bool transB, int m, int n, int k,
                                    float alpha, const float* const* A, int lda, const float* const* B, int ldb,
                                    float beta, float* const* C, int ldc, int batch)
 {
@@ -477,10 +487,10 @@ aclblasStatus_t SgemmBatchedLaunch(hipStream_t st, int cus, bool transA, bool tr
         if (beta == 1.0f) {
             return ACLBLAS_STATUS_SUCCESS;
         }
-        LaunchScaleBatched(st, kOutF32, reinterpret_cast<void* const*>(C), m, n, ldc, beta, batch);
+        LaunchScaleBatched(h->stream, kOutF32, reinterpret_cast<void* const*>(C), m, n, ldc, beta, batch);
         return LaunchStatus("aclblasSgemmBatched");
     }
-    return RunGemm(st, cus, kF32, kF32, kOutF32, transA, transB, m, n, k, alpha, nullptr, lda, nullptr, ldb, beta,
+    return RunGemm(h, kF32, kF32, kOutF32, transA, transB, m, n, k, alpha, nullptr, lda, nullptr, ldb, beta,
                    nullptr, ldc, nullptr, nullptr, reinterpret_cast<const void* const*>(A),
                    reinterpret_cast<const void* const*>(B), reinterpret_cast<void* const*>(C), batch);
 }

@@ -11,22 +11,25 @@
 // problem record with its own tile grid, orders the records by decreasing k
 // (longest tiles first) and copies the table to the workspace through the
 // handle's pinned staging buffer (one asynchronous copy, no synchronisation).
-// One persistent launch of the FP32 MFMA tile kernel (128 x 128 tiles, two
-// workgroups per compute unit) then strides over the concatenated tile lists.
+// One persistent launch of the FP32 MFMA tile kernel then strides over the
+// concatenated tile lists; its tile edge (64 or 128) and workgroups per compute
+// unit come from the tuning table for the call's total tile count (built-in:
+// 128, two per compute unit).
 #include <algorithm>
 #include <vector>
 
 #include "cann_ops_blas.h"
 #include "gemm_kernel.h"
 #include "internal.h"
+#include "tuning.h"
 
 namespace aclblas {
 namespace {
 
 using gemm::GroupedProblem;
 
-constexpr int kTile = 128;
-constexpr int kBlocksPerCu = 2;
+constexpr int kDefaultTile = 128;
+constexpr int kDefaultBlocksPerCu = 2;
 
 bool ValidTrans(aclblasOperation_t t) { return t == ACLBLAS_OP_N || t == ACLBLAS_OP_T; }
 
@@ -124,8 +127,6 @@ extern "C" aclblasStatus_t aclblasSgemmGroupedBatched(
             p.transB = transbArray[g] == ACLBLAS_OP_T ? 1 : 0;
             p.alpha = alpha;
             p.beta = beta;
-            p.tilesM = (p.m + kTile - 1) / kTile;
-            p.tilesN = (p.n + kTile - 1) / kTile;
             p.vecA = Aligned16(p.A, p.lda) ? 1 : 0;
             p.vecB = Aligned16(p.B, p.ldb) ? 1 : 0;
             p.vecC = Aligned16(p.C, p.ldc) ? 1 : 0;
@@ -148,8 +149,18 @@ extern "C" aclblasStatus_t aclblasSgemmGroupedBatched(
     }
     std::stable_sort(probs.begin(), probs.end(),
                      [](const GroupedProblem& a, const GroupedProblem& b) { return a.k > b.k; });
+    const int cus = handle->cu_count > 0 ? handle->cu_count : 256;
+    long long tiles128 = 0;
+    for (const GroupedProblem& p : probs) {
+        tiles128 += static_cast<long long>((p.m + 127) / 128) * ((p.n + 127) / 128);
+    }
+    const tune::Choice c = tune::Get(handle, tune::kOpGrouped, tune::GroupedBucket(tiles128, cus),
+                                     tune::Choice{kDefaultTile, kDefaultBlocksPerCu});
+    const int tile = c.a;
     long long tiles = 0;
     for (GroupedProblem& p : probs) {
+        p.tilesM = (p.m + tile - 1) / tile;
+        p.tilesN = (p.n + tile - 1) / tile;
         p.tileStart = static_cast<int>(tiles);
         tiles += static_cast<long long>(p.tilesM) * p.tilesN;
         if (tiles > 0x7FFFFFFFll) {
@@ -162,10 +173,17 @@ extern "C" aclblasStatus_t aclblasSgemmGroupedBatched(
     if (st != ACLBLAS_STATUS_SUCCESS) {
         return st;
     }
-    const long long cap = static_cast<long long>(handle->cu_count > 0 ? handle->cu_count : 256) * kBlocksPerCu;
+    const long long cap = static_cast<long long>(cus) * c.b;
     const unsigned grid = static_cast<unsigned>(tiles < cap ? tiles : cap);
-    hipLaunchKernelGGL((gemm::GroupedGemmKernel<gemm::kF32, gemm::kF32, gemm::kOutF32, kTile, kTile>), dim3(grid),
-                       dim3(gemm::kThreads), 0, handle->stream, static_cast<const GroupedProblem*>(dev),
-                       static_cast<int>(probs.size()), static_cast<int>(tiles));
+    const auto* table = static_cast<const GroupedProblem*>(dev);
+    if (tile == 64) {
+        hipLaunchKernelGGL((gemm::GroupedGemmKernel<gemm::kF32, gemm::kF32, gemm::kOutF32, 64, 64>), dim3(grid),
+                           dim3(gemm::kThreads), 0, handle->stream, table, static_cast<int>(probs.size()),
+                           static_cast<int>(tiles));
+    } else {
+        hipLaunchKernelGGL((gemm::GroupedGemmKernel<gemm::kF32, gemm::kF32, gemm::kOutF32, 128, 128>), dim3(grid),
+                           dim3(gemm::kThreads), 0, handle->stream, table, static_cast<int>(probs.size()),
+                           static_cast<int>(tiles));
+    }
     return LaunchStatus("aclblasSgemmGroupedBatched");
 }

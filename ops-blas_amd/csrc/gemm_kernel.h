@@ -88,6 +88,7 @@ struct GemmArgs {
     int vecA, vecB;         // 1 when 16-byte vector loads are legal
     int vecC;               // 1 when four consecutive C elements can move as one vector
     int tilesM, tilesN;
+    int group;              // tile rows per L2 group (TileOf)
     // batched form: device arrays of per-problem pointers (A/B/C unused), or null
     const void* const* Aarr;
     const void* const* Barr;
@@ -551,15 +552,14 @@ __device__ __forceinline__ void StoreC4(void* C, size_t idx, bool full, int vali
 // Tile of one problem from its (XCD-ordered) block index.  Workgroups are
 // dispatched round-robin over the 8 XCDs (each with its own L2); the kernel
 // first renumbers blocks so that consecutive numbers run on one XCD, then
-// walks tiles in groups of GROUP tile-rows so that the blocks of one XCD share
-// their A row-panels and B column-panels in that XCD's L2.
-__device__ __forceinline__ void TileOf(int id, int tilesM, int tilesN, int& tm, int& tn)
+// walks tiles in groups of `group` tile-rows so that the blocks of one XCD
+// share their A row-panels and B column-panels in that XCD's L2.
+__device__ __forceinline__ void TileOf(int id, int tilesM, int tilesN, int group, int& tm, int& tn)
 {
-    constexpr int kGroup = 8;
-    const int perGroup = kGroup * tilesN;
+    const int perGroup = group * tilesN;
     const int g = id / perGroup;
-    const int firstM = g * kGroup;
-    const int rowsInGroup = min(kGroup, tilesM - firstM);
+    const int firstM = g * group;
+    const int rowsInGroup = min(group, tilesM - firstM);
     const int inGroup = id - g * perGroup;
     tm = firstM + inGroup % rowsInGroup;
     tn = inGroup / rowsInGroup;
@@ -759,7 +759,7 @@ __global__ __launch_bounds__(kWaveSize * WGM * WGN) void GemmKernel(GemmArgs p)
     const int tilesPer = p.tilesM * p.tilesN;
     const int prob = bid / tilesPer;
     int tm, tn;
-    TileOf(bid - prob * tilesPer, p.tilesM, p.tilesN, tm, tn);
+    TileOf(bid - prob * tilesPer, p.tilesM, p.tilesN, p.group, tm, tn);
 
     TileJob q;
     q.A = static_cast<const uint8_t*>(p.Aarr != nullptr ? p.Aarr[prob] : p.A);
@@ -801,6 +801,10 @@ struct GroupedProblem {
     int vecA, vecB, vecC;
 };
 
+// L2 group height of the grouped kernel's tile walk (problems are small: a
+// few tile rows each).
+constexpr int kGroupedL2Rows = 8;
+
 // Persistent grouped GEMM: the grid strides over the concatenated tile lists
 // of all problems (problems ordered by decreasing k on the host, so the
 // longest tiles start first); each workgroup finds its problem by binary
@@ -824,7 +828,7 @@ __global__ __launch_bounds__(kThreads) void GroupedGemmKernel(const GroupedProbl
         }
         const GroupedProblem& g = probs[lo];
         int tm, tn;
-        TileOf(tile - g.tileStart, g.tilesM, g.tilesN, tm, tn);
+        TileOf(tile - g.tileStart, g.tilesM, g.tilesN, kGroupedL2Rows, tm, tn);
         TileJob q;
         q.A = static_cast<const uint8_t*>(g.A);
         q.B = static_cast<const uint8_t*>(g.B);
@@ -846,32 +850,47 @@ __global__ __launch_bounds__(kThreads) void GroupedGemmKernel(const GroupedProbl
     }
 }
 
-// Tile shape per problem: 256 x 256 for large 1- and 2-byte GEMMs (64 KiB LDS
-// per stage, 8 waves of 128 x 64 so that two waves share each SIMD), 128 x 128
-// (4 waves) for large FP32 ones, 64 x 64 when the 128-tile grid would leave
-// compute units idle.
+// Launch with a tuned choice: tile edge `tile` (256: 256 x 256 over 8 waves of
+// 128 x 64, 64 KiB of LDS per stage, so that two waves share each SIMD; 128 and
+// 64: 2 x 2 waves) and `group` tile rows per L2 group.  A 256 tile is never
+// instantiated for 4-byte operands (it would not fit the register budget).
 template <int KA, int KB, int OK>
-inline void LaunchGemmTyped(hipStream_t st, GemmArgs p, int batch, int cus)
+inline void LaunchGemmTyped(hipStream_t st, GemmArgs p, int batch, int tile, int group)
 {
     constexpr int EB = KindInfo<KA>::bytes;
-    const long long t256 = static_cast<long long>((p.m + 255) / 256) * ((p.n + 255) / 256) * batch;
-    const long long t128 = static_cast<long long>((p.m + 127) / 128) * ((p.n + 127) / 128) * batch;
-    if (EB <= 2 && t256 >= cus) {
-        p.tilesM = (p.m + 255) / 256;
-        p.tilesN = (p.n + 255) / 256;
-        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 256, 256, 2, 4>), dim3(p.tilesM * p.tilesN * batch),
-                           dim3(kWaveSize * 8), 0, st, p);
-    } else if (t128 >= cus) {
+    p.group = group;
+    if constexpr (EB <= 2) {
+        if (tile == 256) {
+            p.tilesM = (p.m + 255) / 256;
+            p.tilesN = (p.n + 255) / 256;
+            hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 256, 256, 2, 4>), dim3(p.tilesM * p.tilesN * batch),
+                               dim3(kWaveSize * 8), 0, st, p);
+            return;
+        }
+    }
+    if (tile == 128) {
         p.tilesM = (p.m + 127) / 128;
         p.tilesN = (p.n + 127) / 128;
         hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 128, 128>), dim3(p.tilesM * p.tilesN * batch), dim3(kThreads),
                            0, st, p);
-    } else {
-        p.tilesM = (p.m + 63) / 64;
-        p.tilesN = (p.n + 63) / 64;
-        hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 64, 64>), dim3(p.tilesM * p.tilesN * batch), dim3(kThreads), 0,
-                           st, p);
+        return;
     }
+    p.tilesM = (p.m + 63) / 64;
+    p.tilesN = (p.n + 63) / 64;
+    hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 64, 64>), dim3(p.tilesM * p.tilesN * batch), dim3(kThreads), 0, st,
+                       p);
+}
+
+// Built-in tile when the tuning table has no entry: 256 for large 1- and
+// 2-byte GEMMs, 128 when its grid fills the compute units, else 64.
+inline int DefaultGemmTile(int elemBytes, int m, int n, int batch, int cus)
+{
+    const long long t256 = static_cast<long long>((m + 255) / 256) * ((n + 255) / 256) * batch;
+    const long long t128 = static_cast<long long>((m + 127) / 128) * ((n + 127) / 128) * batch;
+    if (elemBytes <= 2 && t256 >= cus) {
+        return 256;
+    }
+    return t128 >= cus ? 128 : 64;
 }
 
 }  // namespace gemm

@@ -19,6 +19,7 @@
 #include "cann_ops_blas.h"
 #include "gemm_kernel.h"
 #include "internal.h"
+#include "tuning.h"
 
 namespace aclblas {
 namespace {
@@ -142,14 +143,14 @@ int OutKindScaled(aclDataType t)
 }
 
 template <int KA, int KB>
-void LaunchOut(hipStream_t st, int ok, const GemmArgs& p, int cus)
+void LaunchOut(hipStream_t st, int ok, const GemmArgs& p, int tile, int group)
 {
     if (ok == kOutF32) {
-        LaunchGemmTyped<KA, KB, kOutF32>(st, p, 1, cus);
+        LaunchGemmTyped<KA, KB, kOutF32>(st, p, 1, tile, group);
     } else if (ok == kOutBF16) {
-        LaunchGemmTyped<KA, KB, kOutBF16>(st, p, 1, cus);
+        LaunchGemmTyped<KA, KB, kOutBF16>(st, p, 1, tile, group);
     } else {
-        LaunchGemmTyped<KA, KB, kOutF16>(st, p, 1, cus);
+        LaunchGemmTyped<KA, KB, kOutF16>(st, p, 1, tile, group);
     }
 }
 
@@ -213,15 +214,18 @@ extern "C" aclblasStatus_t aclblasGemmExScaled(
                      ? 1 : 0;
     }
     const int cus = handle->cu_count > 0 ? handle->cu_count : 256;
+    const long long t128 = static_cast<long long>((m + 127) / 128) * ((n + 127) / 128);
+    const tune::Choice c = tune::Get(handle, tune::kOpGemm, tune::GemmBucket(tune::kClassF8Scaled, t128, k, cus),
+                                     tune::Choice{DefaultGemmTile(1, m, n, 1, cus), 8});
     hipStream_t st = handle->stream;
     if (ka == kE4M3S && kb == kE4M3S) {
-        LaunchOut<kE4M3S, kE4M3S>(st, ok, p, cus);
+        LaunchOut<kE4M3S, kE4M3S>(st, ok, p, c.a, c.b);
     } else if (ka == kE4M3S) {
-        LaunchOut<kE4M3S, kE5M2S>(st, ok, p, cus);
+        LaunchOut<kE4M3S, kE5M2S>(st, ok, p, c.a, c.b);
     } else if (kb == kE4M3S) {
-        LaunchOut<kE5M2S, kE4M3S>(st, ok, p, cus);
+        LaunchOut<kE5M2S, kE4M3S>(st, ok, p, c.a, c.b);
     } else {
-        LaunchOut<kE5M2S, kE5M2S>(st, ok, p, cus);
+        LaunchOut<kE5M2S, kE5M2S>(st, ok, p, c.a, c.b);
     }
     return LaunchStatus("aclblasGemmExScaled");
 }

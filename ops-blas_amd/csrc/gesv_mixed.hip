@@ -40,6 +40,7 @@
 #include "cann_ops_blas.h"
 #include "device_common.h"
 #include "internal.h"
+#include "tuning.h"
 
 namespace aclblas {
 namespace {
@@ -868,9 +869,11 @@ extern "C" aclblasStatus_t aclblasSgesvBatchedMixed(
         return st;
     }
     const int cus = handle->cu_count > 0 ? handle->cu_count : 256;
-    // persistent grids: two factor workgroups and three solve workgroups per CU fit (LDS and registers)
-    const unsigned gridF = static_cast<unsigned>(std::min(batchCount, 2 * cus));
-    const unsigned gridS = static_cast<unsigned>(std::min(batchCount, 3 * cus));
+    // persistent grids, workgroups per CU from the tuning table (built-in: two
+    // factor and three solve workgroups, what LDS and registers let one CU hold)
+    const tune::Choice c = tune::Get(handle, tune::kOpMixed, tune::MixedBucket(batchCount, cus), tune::Choice{2, 3});
+    const unsigned gridF = static_cast<unsigned>(std::min(batchCount, c.a * cus));
+    const unsigned gridS = static_cast<unsigned>(std::min(batchCount, c.b * cus));
     const MixedSystem* sys = static_cast<const MixedSystem*>(dev);
     hipLaunchKernelGGL(GesvFactorKernel, dim3(gridF), dim3(kThreads), 0, handle->stream, sys, batchCount);
     hipLaunchKernelGGL(GesvRefineKernel, dim3(gridS), dim3(kThreads), 0, handle->stream, sys, batchCount, tol,

@@ -49,6 +49,9 @@ struct _aclblas_handle {
     // Pinned copies of tables staged while the stream was being captured into
     // a graph (each replay reads its own copy); freed with the handle.
     std::vector<void*> captured_stages;
+    // Per-operation forced launch choice {set, a, b} (aclblasSetTuningOverride);
+    // set == 0 means the measured table decides.
+    int tune_override[3][3] = {};
 };
 
 namespace aclblas {

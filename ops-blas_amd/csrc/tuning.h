@@ -1,0 +1,57 @@
+// Launch choices per size bucket.
+//
+// Every call that has more than one sensible launch shape (GEMM tile and L2
+// tile-group height, grouped-GEMM tile and persistent grid, mixed-solve
+// workgroups per compute unit) maps its problem to a bucket and looks the
+// choice up in a table measured on the target GPU by ops-blas_amd/autotune.py
+// (generated into tuning_table.inc).  Buckets the table does not hold use the
+// built-in fallback of the call.  A handle-scoped override
+// (aclblasSetTuningOverride) forces one choice for every bucket of an
+// operation; the tuner uses it to time each candidate.
+#pragma once
+
+struct _aclblas_handle;
+
+namespace aclblas {
+namespace tune {
+
+enum Op : int {
+    kOpGemm = 0,     // a = tile edge (64, 128, 256), b = tile rows per L2 group (1..64)
+    kOpGrouped = 1,  // a = tile edge (64, 128), b = workgroups per CU of the persistent grid (1..4)
+    kOpMixed = 2,    // a = factor workgroups per CU (1..4), b = solve workgroups per CU (1..4)
+    kNumOps = 3
+};
+
+struct Choice {
+    int a;
+    int b;
+};
+
+// Element classes of the GEMM buckets.
+enum GemmClass : int { kClassF32 = 0, kClass16 = 1, kClassF8 = 2, kClassF8Scaled = 3, kNumGemmClasses = 4 };
+
+constexpr int kGemmSizeBuckets = 5;
+constexpr int kGemmKBuckets = 3;
+constexpr int kGroupedBuckets = 3;
+constexpr int kMixedBuckets = 4;
+
+// GEMM bucket from the element class, the number of 128 x 128 output tiles
+// over the whole call (batch included), k and the compute-unit count.
+int GemmBucket(int cls, long long tiles128, int k, int cus);
+// Grouped GEMM bucket from the total number of 128 x 128 tiles of all problems.
+int GroupedBucket(long long tiles128, int cus);
+// Mixed-solve bucket from the number of systems.
+int MixedBucket(long long count, int cus);
+// Number of buckets of an operation (0 for an unknown one).
+int BucketCount(int op);
+
+// True when `c` is a legal choice for the operation (and, for GEMM, the
+// element class of the bucket).
+bool Valid(int op, int bucket, Choice c);
+
+// Choice for (op, bucket): the handle's override, else the measured table,
+// else `fallback`.
+Choice Get(const _aclblas_handle* h, int op, int bucket, Choice fallback);
+
+}  // namespace tune
+}  // namespace aclblas

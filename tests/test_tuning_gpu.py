@@ -1,0 +1,145 @@
+"""Every launch choice of a tuned call gives the same result, bit for bit.
+
+The tile shape, L2 grouping, persistent grid size and workgroups per CU only
+change which workgroup computes which output; each output sums its products in
+the same order under every choice.  Each case forces every legal choice through
+aclblasSetTuningOverride and compares the outputs with those of the built-in
+choice (ragged shapes, so edge tiles of every tile size are exercised).
+"""
+import numpy as np
+import pytest
+import torch
+
+import aclblas as B
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture()
+def h():
+    handle = B.Handle()
+    assert handle.set_stream(torch.cuda.current_stream().cuda_stream) == 0
+    yield handle.value
+    for op in range(3):
+        B.set_tuning_override(handle.value, op, -1)
+    torch.cuda.synchronize()
+    handle.destroy()
+
+
+def _all_choices_equal(h, op, choices, run, outputs):
+    assert B.set_tuning_override(h, op, -1) == 0
+    run()
+    torch.cuda.synchronize()
+    ref = [o.clone() for o in outputs()]
+    for a, b in choices:
+        assert B.set_tuning_override(h, op, a, b) == 0
+        for o in outputs():
+            o.fill_(float("nan")) if o.is_floating_point() else o.fill_(-7)
+        run()
+        torch.cuda.synchronize()
+        for x, y in zip(ref, outputs()):
+            assert torch.equal(x, y), "choice (%d, %d) differs" % (a, b)
+    assert B.set_tuning_override(h, op, -1) == 0
+
+
+GEMM_CHOICES = [(t, g) for t in (64, 128, 256) for g in (1, 3, 8, 64)]
+
+
+@pytest.mark.parametrize("kind", ["f32", "bf16", "f8", "f8s"])
+@pytest.mark.parametrize("shape", [(300, 257, 129), (520, 130, 700)])
+def test_gemm_choices_agree(h, kind, shape):
+    m, n, k = shape
+    dev = torch.device("cuda")
+    gen = torch.Generator(device="cpu").manual_seed(m + n + k)
+    if kind == "f32":
+        a = (torch.rand(m * k, generator=gen) * 2 - 1).to(dev)
+        b = (torch.rand(k * n, generator=gen) * 2 - 1).to(dev)
+        c0 = torch.randn(m * n, generator=gen).to(dev)
+        c = c0.clone()
+
+        def run():
+            c.copy_(c0)
+            assert B.sgemm(h, B.OP_T, B.OP_N, m, n, k, 1.5, a.data_ptr(), k, b.data_ptr(), k, 0.5, c.data_ptr(),
+                           m) == 0
+        choices = [x for x in GEMM_CHOICES if x[0] != 256]
+    elif kind == "bf16":
+        a = torch.randn(m * k, generator=gen).to(dev).to(torch.bfloat16)
+        b = torch.randn(k * n, generator=gen).to(dev).to(torch.bfloat16)
+        c0 = torch.randn(m * n, generator=gen).to(dev)
+        c = c0.clone()
+
+        def run():
+            c.copy_(c0)
+            assert B.gemm_ex(h, B.OP_N, B.OP_T, m, n, k, 1.0, a.data_ptr(), B.ACL_BF16, m, b.data_ptr(), B.ACL_BF16,
+                             n, 0.25, c.data_ptr(), B.ACL_FLOAT, m, B.COMPUTE["COMPUTE_32F"]) == 0
+        choices = GEMM_CHOICES
+    else:
+        rng = np.random.default_rng(m * k)
+        # finite codes of moderate magnitude: the FP16 results stay finite
+        a = torch.from_numpy(rng.integers(0, 0x50, m * k, dtype=np.uint8)).to(dev)
+        b = torch.from_numpy(rng.integers(0, 0x40, k * n, dtype=np.uint8) |
+                             (rng.integers(0, 2, k * n, dtype=np.uint8) << 7)).to(dev)
+        c = torch.empty(m * n, dtype=torch.float16, device=dev)
+        scale = torch.tensor([0.25, 2.0], device=dev)
+        e4, e5 = B.ACL_FLOAT8_E4M3FN, B.ACL_FLOAT8_E5M2
+        if kind == "f8":
+            def run():
+                assert B.gemm_ex(h, B.OP_N, B.OP_N, m, n, k, 1.0, a.data_ptr(), e4, m, b.data_ptr(), e5, k, 0.0,
+                                 c.data_ptr(), B.ACL_FLOAT16, m, B.COMPUTE["COMPUTE_32F"]) == 0
+        else:
+            def run():
+                assert B.gemm_ex_scaled(h, B.OP_N, B.OP_N, m, n, k, 1.0, a.data_ptr(), e4, m, scale.data_ptr(),
+                                        b.data_ptr(), e4, k, scale.data_ptr() + 4, 0.0, c.data_ptr(),
+                                        B.ACL_FLOAT16, m) == 0
+        choices = GEMM_CHOICES
+    _all_choices_equal(h, B.TUNE_GEMM, choices, run, lambda: [c])
+
+
+def test_grouped_choices_agree(h):
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(4)
+    mnk = rng.integers(1, 40, size=(12, 3)) * 16 + rng.integers(0, 16, size=(12, 3))
+    gen = torch.Generator(device="cpu").manual_seed(4)
+    As = [torch.randn(int(m * k), generator=gen).to(dev) for m, n, k in mnk]
+    Bs = [torch.randn(int(k * n), generator=gen).to(dev) for m, n, k in mnk]
+    C0 = [torch.randn(int(m * n), generator=gen).to(dev) for m, n, k in mnk]
+    Cs = [c.clone() for c in C0]
+    ms, ns, ks = ([int(x) for x in mnk[:, i]] for i in range(3))
+    cnt = len(ms)
+
+    def run():
+        for c, c0 in zip(Cs, C0):
+            c.copy_(c0)
+        assert B.sgemm_grouped_batched(h, [B.OP_N] * cnt, [B.OP_T] * cnt, ms, ns, ks, [1.0] * cnt,
+                                       [x.data_ptr() for x in As], ms, [x.data_ptr() for x in Bs], ns, [0.5] * cnt,
+                                       [x.data_ptr() for x in Cs], ms, [1] * cnt) == 0
+    _all_choices_equal(h, B.TUNE_GROUPED, [(t, w) for t in (64, 128) for w in (1, 2, 3, 4)], run, lambda: Cs)
+
+
+def test_mixed_choices_agree(h):
+    dev = torch.device("cuda")
+    systems = W.conditioned_systems(300, seed=9)
+    offA, offB = [], []
+    oa = ob = 0
+    for s in systems:
+        offA.append(oa)
+        offB.append(ob)
+        oa += s["A"].size
+        ob += s["B"].size
+    dA = torch.from_numpy(np.concatenate([s["A"] for s in systems])).to(dev)
+    dB = torch.from_numpy(np.concatenate([s["B"] for s in systems])).to(dev)
+    dX = torch.empty_like(dB)
+    iters = torch.empty(len(systems), dtype=torch.int32, device=dev)
+    berr = torch.empty(len(systems), dtype=torch.float32, device=dev)
+    ns = [s["n"] for s in systems]
+    rs = [s["nrhs"] for s in systems]
+    pa = [dA.data_ptr() + 4 * o for o in offA]
+    pb = [dB.data_ptr() + 4 * o for o in offB]
+    px = [dX.data_ptr() + 4 * o for o in offB]
+
+    def run():
+        assert B.sgesv_batched_mixed(h, ns, rs, pa, ns, pb, ns, px, ns, 1e-9, 6, iters.data_ptr(), berr.data_ptr(),
+                                     None) == 0
+    _all_choices_equal(h, B.TUNE_MIXED, [(f, s) for f in (1, 2, 4) for s in (1, 3, 4)], run,
+                       lambda: [dX, iters, berr])
