@@ -1,0 +1,261 @@
+// 16-bit (BF16 / FP16) GEMM tile kernel with direct-to-LDS operand loads.
+//
+// One workgroup of 8 waves (2 x 4, each 128 x 64) computes a 256 x 256 tile
+// of C.  The k dimension is consumed in stages of 32; every operand stage
+// (256 x 32 values, 16 KiB) is copied from global memory straight into LDS by
+// 16 one-KiB wave instructions (global_load_lds_dwordx4), with no register
+// staging, so a ring of four LDS stages (128 KiB) keeps three stages of loads
+// in flight while the MFMAs consume the fourth:
+//
+//   issue stages 0, 1, 2
+//   for s: wait for this wave's loads of stage s; barrier;
+//          issue stage s + 3 into the buffer of stage s - 1; compute stage s
+//
+// LDS layouts (both conflict-free for their reads):
+//   K-major (operand stored k-contiguous): 256 rows x 64 bytes (32 k), 16-byte
+//     chunk q of row r at slot q ^ ((r >> 2) & 3); a fragment is one
+//     ds_read_b128 of 8 consecutive k.
+//   M-major (operand stored m- or n-contiguous): 32 rows (one k each) x 512
+//     bytes, chunk c of row kk at slot c ^ MmKey(kk); a fragment is two
+//     transposing reads (ds_read_b64_tr_b16) of 4 k each.
+// The DMA writes 16-byte slots in address order, so each lane picks the global
+// chunk that belongs in its slot.
+//
+// Stage k order: stage 2p + h holds k = 64p + 16 (kk / 8) + 8h + kk % 8 for
+// stage-local kk, so each stage feeds one v_mfma_f32_16x16x32 whose products
+// and order are those of the 128-byte-row kernel (GemmTile): both kernels give
+// bit-identical results.  Whole 64-k pairs are loaded by DMA; a final partial
+// pair is loaded through registers with bounds checks and zero fill.  Rows past
+// the matrix edge read clamped (valid) addresses; their results are not stored.
+#pragma once
+
+#include "lds_dma.h"
+
+namespace aclblas {
+namespace gemm {
+namespace dma {
+
+constexpr int kBM = 256;
+constexpr int kBN = 256;
+constexpr int kKS = 32;                           // k per stage
+constexpr int kStages = 4;
+constexpr int kWGM = 2;
+constexpr int kWGN = 4;
+constexpr int kThreads = kWaveSize * kWGM * kWGN;  // 512
+constexpr int kOpBytes = 256 * kKS * 2;           // one operand stage: 16 KiB
+constexpr int kWaves = kWGM * kWGN;
+
+__device__ __forceinline__ int StageK(int s, int kk)
+{
+    return 64 * (s >> 1) + 16 * (kk >> 3) + 8 * (s & 1) + (kk & 7);
+}
+__device__ __forceinline__ uint32_t KmOffset(int r, int q)
+{
+    return static_cast<uint32_t>(r * 64 + ((q ^ ((r >> 2) & 3)) << 4));
+}
+__device__ __forceinline__ int MmKey(int kk) { return (2 * (kk & 3)) ^ (8 * ((kk >> 3) & 1)); }
+__device__ __forceinline__ uint32_t MmOffset(int kk, int c)
+{
+    return static_cast<uint32_t>(kk * 512 + ((c ^ MmKey(kk)) << 4));
+}
+
+// Per-lane DMA sources of one operand: element pointers of this lane's two
+// 1-KiB instructions (wave w issues instructions w and w + 8) at stage 0, and
+// the element steps from one stage pair to the next and from the first to the
+// second stage of a pair.
+struct DmaSource {
+    const uint16_t* src[2];
+    long long pairStep;
+    long long halfStep;
+
+    // MM: the operand is stored index-contiguous (element (r, k) at X[r + k ld]),
+    // else k-contiguous (X[k + r ld]).  rows: extent of the tile index.
+    template <bool MM>
+    __device__ __forceinline__ void Setup(const uint16_t* X, int ld, int row0, int rows, int wave, int lane)
+    {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int j = wave + kWaves * i;
+            if (MM) {
+                const int kk = 2 * j + (lane >> 5);
+                const int c = (lane & 31) ^ MmKey(kk);
+                const int last = ((rows + 7) & ~7) - 8;  // last 8-aligned chunk inside the column
+                const int r = min(row0 + 8 * c, last);
+                const int k = 16 * (kk >> 3) + (kk & 7);
+                src[i] = X + static_cast<size_t>(k) * ld + r;
+            } else {
+                const int r = min(row0 + 16 * j + (lane >> 2), rows - 1);
+                const int q = (lane & 3) ^ (((16 * j + (lane >> 2)) >> 2) & 3);
+                src[i] = X + static_cast<size_t>(r) * ld + 16 * q;
+            }
+        }
+        pairStep = MM ? 64ll * ld : 64ll;
+        halfStep = MM ? 8ll * ld : 8ll;
+    }
+
+    __device__ __forceinline__ void Issue(int s, uint8_t* buf, int wave) const
+    {
+        const long long off = static_cast<long long>(s >> 1) * pairStep + (s & 1) * halfStep;
+        const uint32_t base = dev::LdsAddr(buf);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            dev::LdsDma16(src[i] + off, __builtin_amdgcn_readfirstlane(base + 1024u * (wave + kWaves * i)));
+        }
+    }
+};
+
+// Register-path load of stage s of one operand (bounds-checked, zero fill):
+// used for the stages of a final partial 64-k pair.
+template <bool MM>
+__device__ __forceinline__ void FillStage(uint8_t* buf, const uint16_t* X, int ld, int row0, int rows, int s, int K,
+                                          int t)
+{
+    for (int e = t; e < 256 * kKS; e += kThreads) {
+        const int r = MM ? (e & 255) : (e >> 5);
+        const int kk = MM ? (e >> 8) : (e & 31);
+        const int k = StageK(s, kk);
+        uint16_t v = 0;
+        if (row0 + r < rows && k < K) {
+            v = MM ? X[static_cast<size_t>(k) * ld + row0 + r] : X[static_cast<size_t>(row0 + r) * ld + k];
+        }
+        const uint32_t off = MM ? MmOffset(kk, r >> 3) + 2u * (r & 7) : KmOffset(r, kk >> 3) + 2u * (kk & 7);
+        *reinterpret_cast<uint16_t*>(buf + off) = v;
+    }
+}
+
+// Byte offset base of a lane's transposing fragment reads in an M-major stage
+// for the wave's fragments starting at tile index wbase: fragment f (16
+// indices, f below the alignment of wbase / 16) reads at TrBase ^ (f << 5),
+// and its second 4 k at that + 2048.
+__device__ __forceinline__ uint32_t TrBase(int wbase, int lane)
+{
+    const int g = lane >> 4;
+    const int j = lane & 15;
+    const int kk = 8 * g + (j >> 2);
+    const int f = MmKey(kk) >> 1;
+    return static_cast<uint32_t>(kk * 512 + (((wbase >> 4) ^ f) << 5) + (j & 3) * 8);
+}
+
+}  // namespace dma
+
+// 256 x 256 tile of one problem (no batch), both operands 16-bit (KA),
+// direct-to-LDS loads.  AMM: op(A) stored m-contiguous (A not transposed);
+// BMM: op(B) stored n-contiguous (B transposed).  The host launches it only
+// when A, B and their leading dimensions are 16-byte aligned.
+template <int KA, int OK, bool AMM, bool BMM>
+__global__ __launch_bounds__(dma::kThreads) void GemmDmaKernel(GemmArgs p)
+{
+    using namespace dma;
+    __shared__ __attribute__((aligned(1024))) uint8_t lds[kStages][2][kOpBytes];
+
+    int bid = blockIdx.x;
+    if (gridDim.x % 8 == 0) {
+        bid = (bid % 8) * (gridDim.x / 8) + bid / 8;
+    }
+    int tm, tn;
+    TileOf(bid, p.tilesM, p.tilesN, p.group, tm, tn);
+    const int m0 = tm * kBM;
+    const int n0 = tn * kBN;
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int wm = (wave % kWGM) * (kBM / kWGM);
+    const int wn = (wave / kWGM) * (kBN / kWGN);
+    const uint16_t* A = static_cast<const uint16_t*>(p.A);
+    const uint16_t* B = static_cast<const uint16_t*>(p.B);
+    const int K = p.k;
+    const int nFull = 2 * (K / 64);
+    const int nst = 2 * ((K + 63) / 64);
+
+    DmaSource da;
+    DmaSource db;
+    da.Setup<AMM>(A, p.lda, m0, p.m, wave, lane);
+    db.Setup<BMM>(B, p.ldb, n0, p.n, wave, lane);
+
+    auto issue = [&](int s) {
+        uint8_t* bufA = lds[s % kStages][0];
+        uint8_t* bufB = lds[s % kStages][1];
+        if (s < nFull) {
+            da.Issue(s, bufA, wave);
+            db.Issue(s, bufB, wave);
+        } else {
+            FillStage<AMM>(bufA, A, p.lda, m0, p.m, s, K, t);
+            FillStage<BMM>(bufB, B, p.ldb, n0, p.n, s, K, t);
+        }
+    };
+
+    // fragment read offsets (bytes within an operand stage)
+    const uint32_t offA = AMM ? TrBase(wm, lane) : KmOffset(wm + (lane & 15), lane >> 4);
+    const uint32_t offB = BMM ? TrBase(wn, lane) : KmOffset(wn + (lane & 15), lane >> 4);
+
+    constexpr int TM = kBM / kWGM / 16;  // 8
+    constexpr int TN = kBN / kWGN / 16;  // 4
+    f32x4 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int s = 0; s < 3 && s < nst; ++s) {
+        issue(s);
+    }
+    for (int s = 0; s < nst; ++s) {
+        // this wave's DMA stages issued after s may stay in flight (4 loads each)
+        const int pending = (s + 1 < nFull ? 1 : 0) + (s + 2 < nFull ? 1 : 0);
+        if (pending == 2) {
+            dev::WaitVmcnt<8>();
+        } else if (pending == 1) {
+            dev::WaitVmcnt<4>();
+        } else {
+            dev::WaitVmcnt<0>();
+        }
+        dev::BarrierLds();
+        if (s + 3 < nst) {
+            issue(s + 3);
+        }
+        const uint8_t* a = lds[s % kStages][0];
+        const uint8_t* b = lds[s % kStages][1];
+        uint4 fb[TN];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            if (BMM) {
+                const uint32_t o = offB ^ (static_cast<uint32_t>(j) << 5);
+                const uint2 lo = dev::ReadTr16(b + o);
+                const uint2 hi = dev::ReadTr16(b + o + 2048);
+                fb[j] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+            } else {
+                fb[j] = *reinterpret_cast<const uint4*>(b + offB + j * 1024);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+            uint4 fa;
+            if (AMM) {
+                const uint32_t o = offA ^ (static_cast<uint32_t>(i) << 5);
+                const uint2 lo = dev::ReadTr16(a + o);
+                const uint2 hi = dev::ReadTr16(a + o + 2048);
+                fa = make_uint4(lo.x, lo.y, hi.x, hi.y);
+            } else {
+                fa = *reinterpret_cast<const uint4*>(a + offA + i * 1024);
+            }
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                Mfma<KA, KA>::RunHalf(fa, fb[j], acc[i][j]);
+            }
+        }
+    }
+
+    TileJob q;
+    q.C = p.C;
+    q.m = p.m;
+    q.n = p.n;
+    q.k = p.k;
+    q.ldc = p.ldc;
+    q.vecC = p.vecC != 0;
+    q.alpha = p.alpha;
+    q.beta = p.beta;
+    StoreTileC<OK, TM, TN, false>(q, acc, m0 + wm, n0 + wn, lane);
+}
+
+}  // namespace gemm
+}  // namespace aclblas

@@ -577,6 +577,53 @@ struct TileJob {
     float alpha, beta;
 };
 
+// Fused epilogue of one wave's TM x TN MFMA tiles whose top-left output
+// element is (row0, col0): C = alpha * acc + beta * C (C is not read when beta
+// is 0); FP8_NAN turns non-finite results into NaN.
+template <int OK, int TM, int TN, bool FP8_NAN>
+__device__ __forceinline__ void StoreTileC(const TileJob& q, const f32x4 (&acc)[TM][TN], int row0, int col0, int lane)
+{
+    const float alpha = q.alpha;
+    const float beta = q.beta;
+    const int fr = lane & 15;
+    const int rq = (lane >> 4) * 4;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int col = col0 + j * 16 + fr;
+            const int row = row0 + i * 16 + rq;
+            if (col >= q.n || row >= q.m) {
+                continue;
+            }
+            const size_t base = static_cast<size_t>(col) * q.ldc + row;
+            float v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v[e] = alpha * acc[i][j][e];
+            }
+            const bool full = q.vecC && row + 4 <= q.m;
+            if (beta != 0.0f) {
+                float c[4];
+                LoadC4<OK>(q.C, base, full, q.m - row, c);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    v[e] = v[e] + beta * c[e];
+                }
+            }
+            if (FP8_NAN) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    if (!__builtin_isfinite(v[e])) {
+                        v[e] = __builtin_nanf("");
+                    }
+                }
+            }
+            StoreC4<OK>(q.C, base, full, q.m - row, v);
+        }
+    }
+}
+
 // Computes C[m0:m0+BM, n0:n0+BN] = alpha * op(A) op(B) + beta * C for one job
 // with the LDS stages sA/sB.  FP8 non-finite results become NaN (reference
 // test/gemm/gemm_ex/gemm_ex_golden.h, QuantizeGoldenOutput: the reference's FP8
@@ -702,45 +749,7 @@ __device__ __forceinline__ void GemmTile(const TileJob& q, int m0, int n0, uint8
         __syncthreads();
     }
 
-    // ---- fused epilogue: C = alpha * acc + beta * C ----
-    const float alpha = q.alpha;
-    const float beta = q.beta;
-    const int rq = (lane >> 4) * 4;
-#pragma unroll
-    for (int i = 0; i < TM; ++i) {
-#pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            const int col = n0 + wn + j * 16 + fr;
-            const int row = m0 + wm + i * 16 + rq;
-            if (col >= q.n || row >= q.m) {
-                continue;
-            }
-            const size_t base = static_cast<size_t>(col) * q.ldc + row;
-            float v[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                v[e] = alpha * acc[i][j][e];
-            }
-            const bool full = q.vecC && row + 4 <= q.m;
-            if (beta != 0.0f) {
-                float c[4];
-                LoadC4<OK>(q.C, base, full, q.m - row, c);
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    v[e] = v[e] + beta * c[e];
-                }
-            }
-            if (kFp8) {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    if (!__builtin_isfinite(v[e])) {
-                        v[e] = __builtin_nanf("");
-                    }
-                }
-            }
-            StoreC4<OK>(q.C, base, full, q.m - row, v);
-        }
-    }
+    StoreTileC<OK, TM, TN, kFp8>(q, acc, m0 + wm, n0 + wn, lane);
 }
 
 // One problem (or a uniform batch through pointer arrays): one workgroup per
@@ -850,6 +859,14 @@ __global__ __launch_bounds__(kThreads) void GroupedGemmKernel(const GroupedProbl
     }
 }
 
+}  // namespace gemm
+}  // namespace aclblas
+
+#include "gemm_dma.h"
+
+namespace aclblas {
+namespace gemm {
+
 // Launch with a tuned choice: tile edge `tile` (256: 256 x 256 over 8 waves of
 // 128 x 64, 64 KiB of LDS per stage, so that two waves share each SIMD; 128 and
 // 64: 2 x 2 waves) and `group` tile rows per L2 group.  A 256 tile is never
@@ -859,6 +876,25 @@ inline void LaunchGemmTyped(hipStream_t st, GemmArgs p, int batch, int tile, int
 {
     constexpr int EB = KindInfo<KA>::bytes;
     p.group = group;
+    if constexpr (KA == kBF16 || KA == kF16) {
+        // direct-to-LDS kernel: one problem, 16-byte aligned operands
+        if (tile == 256 && p.Aarr == nullptr && p.vecA != 0 && p.vecB != 0) {
+            p.tilesM = (p.m + 255) / 256;
+            p.tilesN = (p.n + 255) / 256;
+            const dim3 grid(p.tilesM * p.tilesN);
+            const dim3 block(dma::kThreads);
+            if (p.transA == 0 && p.transB == 0) {
+                hipLaunchKernelGGL((GemmDmaKernel<KA, OK, true, false>), grid, block, 0, st, p);
+            } else if (p.transA == 0) {
+                hipLaunchKernelGGL((GemmDmaKernel<KA, OK, true, true>), grid, block, 0, st, p);
+            } else if (p.transB == 0) {
+                hipLaunchKernelGGL((GemmDmaKernel<KA, OK, false, false>), grid, block, 0, st, p);
+            } else {
+                hipLaunchKernelGGL((GemmDmaKernel<KA, OK, false, true>), grid, block, 0, st, p);
+            }
+            return;
+        }
+    }
     if constexpr (EB <= 2) {
         if (tile == 256) {
             p.tilesM = (p.m + 255) / 256;

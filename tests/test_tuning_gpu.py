@@ -143,3 +143,51 @@ def test_mixed_choices_agree(h):
                                      None) == 0
     _all_choices_equal(h, B.TUNE_MIXED, [(f, s) for f in (1, 2, 4) for s in (1, 3, 4)], run,
                        lambda: [dX, iters, berr])
+
+
+def _round8(x):
+    return (x + 7) // 8 * 8
+
+
+@pytest.mark.parametrize("kinds", [("BF16", "FP32"), ("FP16", "FP16"), ("BF16", "BF16")])
+@pytest.mark.parametrize("trans", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("shape", [(512, 384, 320), (264, 520, 129), (40, 33, 64), (300, 257, 32)])
+def test_direct_to_lds_kernel_matches_register_staged(h, kinds, trans, shape):
+    """256 tiles of 16-bit operands with 16-byte aligned leading dimensions run the
+    direct-to-LDS kernel; 128 tiles run the register-staged kernel.  Same products,
+    same order: the outputs must agree bit for bit, and with an FP32 reference."""
+    m, n, k = shape
+    at, ct = kinds
+    ta, tb = trans
+    dev = torch.device("cuda")
+    tdt = {"BF16": torch.bfloat16, "FP16": torch.float16, "FP32": torch.float32}
+    rowsA, colsA = (k, m) if ta else (m, k)
+    rowsB, colsB = (n, k) if tb else (k, n)
+    lda, ldb, ldc = _round8(rowsA), _round8(rowsB), m + 3
+    gen = torch.Generator(device="cpu").manual_seed(m * 7 + n * 3 + k + 31 * ta + 17 * tb)
+    A = torch.randn(lda * colsA, generator=gen).to(tdt[at]).to(dev)
+    Bm = torch.randn(ldb * colsB, generator=gen).to(tdt[at]).to(dev)
+    C0 = torch.randn(ldc * n, generator=gen).to(tdt[ct]).to(dev)
+    outs = []
+    for tile in (256, 128):
+        assert B.set_tuning_override(h, B.TUNE_GEMM, tile, 8) == 0
+        C = C0.clone()
+        assert B.gemm_ex(h, B.OP_T if ta else B.OP_N, B.OP_T if tb else B.OP_N, m, n, k, 0.75, A.data_ptr(),
+                         B.DTYPES[at], lda, Bm.data_ptr(), B.DTYPES[at], ldb, -0.5, C.data_ptr(), B.DTYPES[ct], ldc,
+                         B.COMPUTE["COMPUTE_32F"]) == 0
+        torch.cuda.synchronize()
+        outs.append(C)
+    assert torch.equal(outs[0], outs[1]), "direct-to-LDS and register-staged kernels differ"
+    # FP32 reference (column-major views)
+    Af = A.float().cpu().reshape(colsA, lda)[:, :rowsA].t()      # rowsA x colsA
+    Bf = Bm.float().cpu().reshape(colsB, ldb)[:, :rowsB].t()
+    opA = Af.t() if ta else Af
+    opB = Bf.t() if tb else Bf
+    ref = 0.75 * (opA.double() @ opB.double()) - 0.5 * C0.float().cpu().reshape(n, ldc)[:, :m].t().double()
+    got = outs[0].float().cpu().reshape(n, ldc)[:, :m].t().double()
+    tol = {"FP32": 1e-5, "FP16": 2e-3, "BF16": 1e-2}[ct] * max(1.0, ref.abs().max().item())
+    assert (got - ref).abs().max().item() <= tol
+    # padding rows of C untouched
+    pad_before = C0.cpu().reshape(n, ldc)[:, m:]
+    pad_after = outs[0].cpu().reshape(n, ldc)[:, m:]
+    assert torch.equal(pad_before, pad_after)
