@@ -8,8 +8,9 @@
 // in flight while the MFMAs consume the fourth:
 //
 //   issue stages 0, 1, 2
-//   for s: wait for this wave's loads of stage s; barrier;
-//          issue stage s + 3 into the buffer of stage s - 1; compute stage s
+//   for s: wait for this wave's loads of stage s; barrier; compute stage s,
+//          issuing stage s + 3 into the buffer of stage s - 1 after its first
+//          eight MFMAs
 //
 // LDS layouts (both conflict-free for their reads):
 //   K-major (operand stored k-contiguous): 256 rows x 64 bytes (32 k), 16-byte
@@ -210,9 +211,9 @@ __global__ __launch_bounds__(dma::kThreads) void GemmDmaKernel(GemmArgs p)
             dev::WaitVmcnt<0>();
         }
         dev::BarrierLds();
-        if (s + 3 < nst) {
-            issue(s + 3);
-        }
+        // raised wave priority over the MFMA block: the SIMD's other wave
+        // (address arithmetic, waits) yields issue slots to it
+        dev::SetPriority<1>();
         const uint8_t* a = lds[s % kStages][0];
         const uint8_t* b = lds[s % kStages][1];
         uint4 fb[TN];
@@ -242,7 +243,12 @@ __global__ __launch_bounds__(dma::kThreads) void GemmDmaKernel(GemmArgs p)
             for (int j = 0; j < TN; ++j) {
                 Mfma<KA, KA>::RunHalf(fa, fb[j], acc[i][j]);
             }
+            // the next stage's loads go out once the MFMA pipe is fed
+            if (i == 1 && s + 3 < nst) {
+                issue(s + 3);
+            }
         }
+        dev::SetPriority<0>();
     }
 
     TileJob q;

@@ -42,6 +42,14 @@ __device__ __forceinline__ void WaitVmcnt()
     asm volatile("s_waitcnt vmcnt(%0)" : : "n"(N) : "memory");
 }
 
+// Issue priority of this wave relative to the other waves of its SIMD (0..3).
+template <int P>
+__device__ __forceinline__ void SetPriority()
+{
+    static_assert(P >= 0 && P <= 3, "priority range");
+    asm volatile("s_setprio %0" : : "n"(P));
+}
+
 // Workgroup barrier after this wave's LDS writes have completed.
 __device__ __forceinline__ void BarrierLds()
 {
