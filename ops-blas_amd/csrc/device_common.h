@@ -76,15 +76,15 @@ __device__ __forceinline__ uint32_t SegReduce(uint32_t v)
         v = op(v, Dpp<0x140>(v));
     }
     if constexpr (S >= 32) {
-        const uint32_t r0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 0));
-        const uint32_t r1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 16));
-        const uint32_t r2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 32));
-        const uint32_t r3 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 48));
-        if constexpr (S == 32) {
-            v = (threadIdx.x & 32) ? op(r2, r3) : op(r0, r1);
-        } else {
-            v = op(op(r0, r1), op(r2, r3));
-        }
+        // v_permlane16_swap: the odd 16-lane rows of the first copy trade places
+        // with the even rows of the second, so every lane sees its partner row
+        const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        v = op(p[0], p[1]);
+    }
+    if constexpr (S == 64) {
+        // v_permlane32_swap: the same between the two 32-lane halves
+        const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        v = op(p[0], p[1]);
     }
     return v;
 }

@@ -7,7 +7,7 @@
 // j >= n are padding: zero, never pivot candidates, never updated, never
 // stored.  Per column k:
 //   1. pivot search: a 32-bit |a_rk| key, reduced over the segment with DPP row
-//      operations (plus four v_readlane across 16-lane rows for S >= 32); exact
+//      operations (plus v_permlane16/32_swap across 16-lane rows); exact
 //      ties between candidates fall back to a second reduction that takes the
 //      lowest current position;
 //   2. the winning lane writes its row and its position to a per-segment LDS
@@ -42,9 +42,11 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void GetrfWaveKernel(
     const int b = (blockIdx.x * kWavesPerBlock + wave) * kMpw + seg;
     const bool live = b < batch;
     const bool real = live && r < n;
-    float* slot = srow[wave][seg];
+    float* slot = static_cast<float*>(__builtin_assume_aligned(srow[wave][seg], 16));
 
-    float* A = live ? Aarray[b] : nullptr;
+    // global address space: plain global loads and stores instead of flat ones
+    typedef __attribute__((address_space(1))) float GlobalFloat;
+    GlobalFloat* A = live ? (GlobalFloat*)(Aarray[b]) : nullptr;
     float a[S];
     {
         const int jlim = real ? n : 0;
