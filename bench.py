@@ -33,6 +33,7 @@ METRIC = "batched LU solves/s BF16+refine vs FP32 at equal backward error; GEMM 
 BATCH, N, NRHS = 4096, 32, 1
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
+FP8_PEAK_TFLOPS = 5000.0
 FP32_PEAK_TFLOPS = 157.3
 # algorithmic bytes per system: getrf reads A and writes LU (n*n*4 each), writes
 # n pivots and one info word, reads one 8-byte pointer
@@ -242,6 +243,38 @@ def bench_gemm(torch, B, W, h, timer):
     ms = timer(f, 10)
     tf = 2 * m ** 3 / (ms * 1e-3) / 1e12
     out["bf16_fp32_8192"] = {"ms": ms, "tflops": tf, "frac_of_peak": tf / BF16_PEAK_TFLOPS}
+    # FP8 E4M3 with per-tensor amax scales, 8192^3: quantise A and B (timed separately), scaled GEMM -> BF16
+    af = torch.randn(m * m, device=dev)
+    bf = torch.randn(m * m, device=dev)
+    qa = torch.empty(m * m, dtype=torch.uint8, device=dev)
+    qb = torch.empty(m * m, dtype=torch.uint8, device=dev)
+    sc = torch.empty(2, device=dev)
+    cb = torch.empty(m * m, dtype=torch.bfloat16, device=dev)
+    e4 = B.ACL_FLOAT8_E4M3FN
+
+    def quant():
+        assert B.quantize_fp8(h.value, m * m, af.data_ptr(), e4, qa.data_ptr(), sc.data_ptr()) == 0
+        assert B.quantize_fp8(h.value, m * m, bf.data_ptr(), e4, qb.data_ptr(), sc.data_ptr() + 4) == 0
+
+    f = lambda: B.gemm_ex_scaled(h.value, B.OP_N, B.OP_N, m, m, m, 1.0, qa.data_ptr(), e4, m, sc.data_ptr(),
+                                 qb.data_ptr(), e4, m, sc.data_ptr() + 4, 0.0, cb.data_ptr(), B.ACL_BF16, m)
+    quant()
+    f()
+    q_ms = timer(quant, 10)
+    ms = timer(f, 10)
+    tf = 2 * m ** 3 / (ms * 1e-3) / 1e12
+    out["fp8_scaled_8192"] = {"ms": ms, "tflops": tf, "frac_of_peak": tf / FP8_PEAK_TFLOPS,
+                              "quantize_both_ms": q_ms}
+    del af, bf, qa, qb, cb
+    # FP32 8192^3 (aclblasSgemm), N(0,1)
+    a = torch.randn(m * m, device=dev)
+    b = torch.randn(m * m, device=dev)
+    f = lambda: B.sgemm(h.value, B.OP_N, B.OP_N, m, m, m, 1.0, a.data_ptr(), m, b.data_ptr(), m, 0.0,
+                        c.data_ptr(), m)
+    f()
+    ms = timer(f, 5)
+    tf = 2 * m ** 3 / (ms * 1e-3) / 1e12
+    out["fp32_8192"] = {"ms": ms, "tflops": tf, "frac_of_peak": tf / FP32_PEAK_TFLOPS}
     del a, b, c
     # grouped FP32: 64 groups, m, n, k in {64, 80, ..., 2048}, seed 0, one problem per group
     rng = np.random.default_rng(0)
@@ -324,6 +357,13 @@ def main():
     if args.profile_gemm:
         h = B.Handle()
         h.set_stream(torch.cuda.current_stream().cuda_stream)
+        if args.calibrate_copy > 0:
+            src = torch.ones(args.calibrate_copy * (1 << 18), device=dev)
+            dst = torch.empty_like(src)
+            for _ in range(5):
+                torch.neg(src, out=dst)
+            torch.cuda.synchronize()
+            del src, dst
         m = 8192
         a = torch.randn(m * m, device=dev).to(torch.bfloat16)
         b = torch.randn(m * m, device=dev).to(torch.bfloat16)
