@@ -234,7 +234,9 @@ aclblasStatus_t aclblasSgesvBatchedMixed(
  *     a = output tile edge 64 / 128 / 256 (256: 1- and 2-byte operands only),
  *     b = tile rows walked together per L2 group, 1..64;
  *   1 grouped GEMM: a = tile edge 64 / 128, b = persistent workgroups per CU 1..4;
- *   2 mixed solve: a = factor workgroups per CU 1..4, b = solve workgroups per CU 1..4.
+ *   2 mixed solve: a = factor workgroups per CU 1..4, b = solve workgroups per CU 1..4;
+ *   3 batched LU (aclblasSgetrfBatched, n <= 64): a = matrix rows per lane 1 or 2
+ *     (2 needs n > 8), b ignored.
  * Results do not depend on the choice (every output element sums its products
  * in the same order under every tile shape and grid). */
 
@@ -250,7 +252,8 @@ aclblasStatus_t aclblasGetTuningChoice(aclblasHandle_t handle, int op, int bucke
 /* Bucket of a problem: op 0 takes cls (0 FP32, 1 BF16/FP16, 2 FP8 under
  * GemmEx, 3 FP8 under GemmExScaled), units = 128 x 128 output tiles over the
  * call and k; op 1 takes units = 128 x 128 tiles over all problems; op 2 takes
- * units = number of systems.  cls and k are ignored for ops 1 and 2. */
+ * units = number of systems; op 3 takes cls = the order n (1..64) and
+ * units = the batch count.  cls and k are ignored for ops 1 and 2, k for op 3. */
 aclblasStatus_t aclblasTuningBucket(aclblasHandle_t handle, int op, int cls, long long units, int k, int* bucket);
 
 #ifdef __cplusplus

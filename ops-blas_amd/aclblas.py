@@ -71,7 +71,7 @@ _SIGNATURES = {
 }
 
 # tuning operations (aclblasSetTuningOverride)
-TUNE_GEMM, TUNE_GROUPED, TUNE_MIXED = 0, 1, 2
+TUNE_GEMM, TUNE_GROUPED, TUNE_MIXED, TUNE_LU = 0, 1, 2, 3
 # GEMM bucket element classes
 CLASS_F32, CLASS_16, CLASS_F8, CLASS_F8_SCALED = 0, 1, 2, 3
 

@@ -20,6 +20,9 @@ Buckets and representative problems:
     64/128 x 1..4 workgroups per CU.
   mixed solve, 4 buckets: 128, 512, 2048, 10000 systems, n 16..256, nrhs
     1..16, cond 1e2..1e6; candidates 1..2 factor x 1..4 solve workgroups per CU.
+  batched LU (n <= 64), 16 buckets: n = 16, 32, 64 with batches giving 1, 3,
+    6 and 12 one-row-per-lane waves per SIMD (n <= 8 has one choice);
+    candidates 1 or 2 matrix rows per lane.
 After writing the table, rebuild the library (make -C ops-blas_amd).
 """
 import argparse
@@ -72,7 +75,7 @@ class Tuner:
         self.report = []    # markdown lines
 
     def close(self):
-        for op in range(3):
+        for op in range(4):
             B.set_tuning_override(self.h.value, op, -1)
         self.h.destroy()
 
@@ -235,6 +238,34 @@ class Tuner:
             cands = [(f, s) for f in (1, 2) for s in (1, 2, 3, 4)]
             self._sweep(B.TUNE_MIXED, bucket, cands, run, lambda: [dX, iters], "%d mixed systems" % count)
 
+    # ------------------------------------------------------------ batched LU
+    def tune_lu(self):
+        torch = self.torch
+        dev = self.dev
+        h = self.h.value
+        simds = 4 * self.cus
+        for S in (16, 32, 64):
+            for want, waves in enumerate((simds, 3 * simds, 6 * simds, 12 * simds)):
+                batch = waves * 64 // S
+                bucket = self.bucket(B.TUNE_LU, S, batch)
+                if bucket != (S.bit_length() - 4) * 4 + want:
+                    raise RuntimeError("LU representative n=%d batch=%d: bucket %d" % (S, batch, bucket))
+                A0, _ = W.lu_batch(batch, S, 1, seed=S + want)
+                dA0 = torch.from_numpy(A0).to(dev)
+                LU = dA0.clone()
+                piv = torch.empty((batch, S), dtype=torch.int32, device=dev)
+                info = torch.empty(batch, dtype=torch.int32, device=dev)
+                ptrs = B.ptr_array(LU)
+
+                def run():
+                    LU.copy_(dA0)
+                    assert B.sgetrf_batched(h, S, ptrs.data_ptr(), S, piv.data_ptr(), info.data_ptr(), batch) == 0
+
+                self._sweep(B.TUNE_LU, bucket, [(1, 1), (2, 1)], run, lambda: [LU, piv, info],
+                            "LU n=%d batch=%d" % (S, batch))
+                del dA0, LU, piv, info, ptrs
+                torch.cuda.empty_cache()
+
     # ------------------------------------------------------------------ output
     def write_table(self, path, keep_from=None):
         """Writes the measured rows plus, from `keep_from`, the rows of operations not tuned in this run."""
@@ -279,7 +310,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--table", default=os.path.join(HERE, "csrc", "tuning_table.inc"))
     ap.add_argument("--report", default=None, help="markdown summary of every measurement")
-    ap.add_argument("--ops", default="gemm,grouped,mixed", help="comma list of gemm, grouped, mixed")
+    ap.add_argument("--ops", default="gemm,grouped,mixed,lu", help="comma list of gemm, grouped, mixed, lu")
     ap.add_argument("--quick", action="store_true", help="one L2 group height, fewer repetitions")
     ap.add_argument("--keep-from", default=os.path.join(HERE, "csrc", "tuning_table.inc"),
                     help="table whose rows for operations not tuned in this run are carried over")
@@ -298,6 +329,8 @@ def main(argv=None):
             tuner.tune_grouped()
         if "mixed" in ops:
             tuner.tune_mixed()
+        if "lu" in ops:
+            tuner.tune_lu()
     finally:
         tuner.close()
     tuner.write_table(args.table, args.keep_from)

@@ -10,8 +10,10 @@ struct _aclblas_handle;
 
 namespace aclblas {
 
-// LU with partial pivoting (piv == nullptr: no pivoting), getrf.hip.
-aclblasStatus_t GetrfBatchedLaunch(hipStream_t st, int n, float* const* A, int lda, int* piv, int* info, int batch);
+// LU with partial pivoting (piv == nullptr: no pivoting) on the handle's
+// stream with its tuned launch shape, getrf.hip.
+aclblasStatus_t GetrfBatchedLaunch(const _aclblas_handle* h, int n, float* const* A, int lda, int* piv, int* info,
+                                   int batch);
 
 // op(A) X = B with LU factors, B overwritten (trans: A^T), getrs.hip.
 aclblasStatus_t GetrsBatchedLaunch(hipStream_t st, bool trans, int n, int nrhs, const float* const* A, int lda,

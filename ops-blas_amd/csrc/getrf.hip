@@ -11,8 +11,9 @@
 // rows whose position is >= k, and at the end each row is stored once at its
 // final position.  The LAPACK-format pivot vector (ipiv[k] = position of the
 // chosen row at step k, 1-based) falls out of the position bookkeeping.
-//   n <= 64  : one wave per matrix, one lane per row, the pivot row broadcast
-//              with v_readlane (no LDS, no barriers).
+//   n <= 64  : one segment of a wave per matrix, one or two rows per lane
+//              (tuned per size bucket), the pivot row broadcast through a
+//              per-segment LDS slot (no barriers).
 //   n <= 256 : one workgroup per matrix, 2-D block-cyclic ownership (RPT rows x
 //              CPT columns per thread), pivot row and multipliers exchanged
 //              through LDS, two barriers per column.
@@ -25,6 +26,7 @@
 #include "drivers.h"
 #include "internal.h"
 #include "lapack_kernels.h"
+#include "tuning.h"
 
 namespace aclblas {
 namespace {
@@ -115,10 +117,18 @@ __global__ __launch_bounds__(kGlobalThreads) void GetrfGlobalKernel(
 }
 
 template <bool PIVOT>
-void LaunchGetrf(hipStream_t st, float* const* A, int n, int lda, int* ipiv, int* info, int batch)
+void LaunchGetrf(const _aclblas_handle* h, float* const* A, int n, int lda, int* ipiv, int* info, int batch)
 {
+    const hipStream_t st = h->stream;
     if (n <= 64) {
-        LaunchGetrfWave<PIVOT>(st, A, n, lda, ipiv, info, batch);
+        // two rows per lane halve the per-matrix instruction count but also the
+        // waves: built in only while that still leaves two waves per SIMD
+        const int cus = h->cu_count > 0 ? h->cu_count : 256;
+        const int S = n <= 8 ? 8 : (n <= 16 ? 16 : (n <= 32 ? 32 : 64));
+        const long long wavesTwoRows = static_cast<long long>(batch) * S / 128;
+        const int builtIn = (S > 8 && wavesTwoRows >= 8ll * cus) ? 2 : 1;
+        const tune::Choice c = tune::Get(h, tune::kOpLu, tune::LuBucket(S, batch, cus), tune::Choice{builtIn, 1});
+        LaunchGetrfWave<PIVOT>(st, A, n, lda, ipiv, info, batch, c.a);
     } else if (n <= 256) {
         LaunchGetrfBlock<PIVOT>(st, A, n, lda, ipiv, info, batch);
     } else {
@@ -152,12 +162,13 @@ aclblasStatus_t ValidateGetrf(int n, float* const* A, int lda, const int* piv, c
 }  // namespace
 
 // Internal entry shared with the solve/refinement drivers.
-aclblasStatus_t GetrfBatchedLaunch(hipStream_t st, int n, float* const* A, int lda, int* piv, int* info, int batch)
+aclblasStatus_t GetrfBatchedLaunch(const _aclblas_handle* h, int n, float* const* A, int lda, int* piv, int* info,
+                                   int batch)
 {
     if (piv != nullptr) {
-        LaunchGetrf<true>(st, A, n, lda, piv, info, batch);
+        LaunchGetrf<true>(h, A, n, lda, piv, info, batch);
     } else {
-        LaunchGetrf<false>(st, A, n, lda, piv, info, batch);
+        LaunchGetrf<false>(h, A, n, lda, piv, info, batch);
     }
     return LaunchStatus("aclblasSgetrfBatched");
 }
@@ -186,5 +197,5 @@ extern "C" aclblasStatus_t aclblasSgetrfBatched(
     if (!HandleHasDevice(handle)) {
         return ACLBLAS_STATUS_EXECUTION_FAILED;
     }
-    return GetrfBatchedLaunch(handle->stream, n, Aarray, lda, PivotArray, infoArray, batchSize);
+    return GetrfBatchedLaunch(handle, n, Aarray, lda, PivotArray, infoArray, batchSize);
 }

@@ -51,7 +51,7 @@ struct _aclblas_handle {
     std::vector<void*> captured_stages;
     // Per-operation forced launch choice {set, a, b} (aclblasSetTuningOverride);
     // set == 0 means the measured table decides.
-    int tune_override[3][3] = {};
+    int tune_override[4][3] = {};
 };
 
 namespace aclblas {

@@ -6,9 +6,9 @@
 
 namespace aclblas {
 
-// LU, n <= 64 (getrf_wave.hip)
+// LU, n <= 64 (getrf_wave.hip); rows: matrix rows per lane (1 or 2; 1 for n <= 8)
 template <bool PIVOT>
-void LaunchGetrfWave(hipStream_t st, float* const* A, int n, int lda, int* ipiv, int* info, int batch);
+void LaunchGetrfWave(hipStream_t st, float* const* A, int n, int lda, int* ipiv, int* info, int batch, int rows);
 
 // LU, 64 < n <= 256 (getrf_block.hip)
 template <bool PIVOT>

@@ -56,6 +56,15 @@ int MixedBucket(long long count, int cus)
     return SizeBucket(count, cus, kEdges, kMixedBuckets);
 }
 
+int LuBucket(int S, long long batch, int cus)
+{
+    const int si = S <= 8 ? 0 : (S <= 16 ? 1 : (S <= 32 ? 2 : 3));
+    // waves of a one-row-per-lane launch: < 2, < 4, < 8 per SIMD (4 SIMDs per CU), more
+    static const long long kEdges[4] = {32, 64, 128, 0};
+    const int w = SizeBucket(batch * S / 64, cus, kEdges, 4);
+    return si * 4 + w;
+}
+
 int BucketCount(int op)
 {
     switch (op) {
@@ -65,6 +74,8 @@ int BucketCount(int op)
             return kGroupedBuckets;
         case kOpMixed:
             return kMixedBuckets;
+        case kOpLu:
+            return kLuBuckets;
         default:
             return 0;
     }
@@ -85,6 +96,8 @@ bool Valid(int op, int bucket, Choice c)
             return (c.a == 64 || c.a == 128) && c.b >= 1 && c.b <= 4;
         case kOpMixed:
             return c.a >= 1 && c.a <= 4 && c.b >= 1 && c.b <= 4;
+        case kOpLu:
+            return c.a == 1 || (c.a == 2 && bucket >= 4);
         default:
             return false;
     }
@@ -175,6 +188,12 @@ extern "C" aclblasStatus_t aclblasTuningBucket(aclblasHandle_t handle, int op, i
             return ACLBLAS_STATUS_SUCCESS;
         case tune::kOpMixed:
             *bucket = tune::MixedBucket(units, cus);
+            return ACLBLAS_STATUS_SUCCESS;
+        case tune::kOpLu:
+            if (cls < 1 || cls > 64) {
+                return ACLBLAS_STATUS_INVALID_VALUE;
+            }
+            *bucket = tune::LuBucket(cls <= 8 ? 8 : (cls <= 16 ? 16 : (cls <= 32 ? 32 : 64)), units, cus);
             return ACLBLAS_STATUS_SUCCESS;
         default:
             return ACLBLAS_STATUS_INVALID_VALUE;

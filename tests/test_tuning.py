@@ -51,10 +51,22 @@ def test_grouped_and_mixed_buckets(h):
         [0, 0, 1, 2, 2, 3]
 
 
+def test_lu_buckets(h):
+    # S index x waves of a one-row-per-lane launch (batch * S / 64) per SIMD: < 2, < 4, < 8, more
+    simds = 4 * CUS
+    for n, si in ((1, 0), (8, 0), (9, 1), (16, 1), (17, 2), (32, 2), (33, 3), (64, 3)):
+        S = 8 << si
+        for w, waves in enumerate((simds, 3 * simds, 6 * simds, 12 * simds)):
+            assert _bucket(h, B.TUNE_LU, n, waves * 64 // S) == si * 4 + w
+        assert _bucket(h, B.TUNE_LU, n, 2 * simds * 64 // S) == si * 4 + 1
+    assert B.tuning_bucket(h, B.TUNE_LU, 0, 10)[0] == status_code("INVALID_VALUE")
+    assert B.tuning_bucket(h, B.TUNE_LU, 65, 10)[0] == status_code("INVALID_VALUE")
+
+
 def test_bucket_argument_checks(h):
     inv = status_code("INVALID_VALUE")
     assert B.tuning_bucket(None, 0, 0, 1, 1)[0] == status_code("HANDLE_IS_NULLPTR")
-    assert B.tuning_bucket(h, 3, 0, 1, 1)[0] == inv
+    assert B.tuning_bucket(h, 4, 0, 1, 1)[0] == inv
     assert B.tuning_bucket(h, 0, 4, 1, 1)[0] == inv
     assert B.tuning_bucket(h, 0, 0, -1, 1)[0] == inv
     assert B.tuning_bucket(h, 0, 0, 1, -1)[0] == inv
@@ -75,7 +87,7 @@ def test_table_rows_are_legal_and_used(h):
     for op, bucket, a, b in rows:
         assert (op, bucket) not in seen, "duplicate row"
         seen.add((op, bucket))
-        assert 0 <= op < 3
+        assert 0 <= op < 4
         st, ga, gb = B.get_tuning_choice(h, op, bucket)
         assert st == 0
         assert (ga, gb) == (a, b), "row %r not honoured" % ((op, bucket, a, b),)
@@ -84,8 +96,10 @@ def test_table_rows_are_legal_and_used(h):
             assert not (a == 256 and bucket // 15 == B.CLASS_F32)
         elif op == B.TUNE_GROUPED:
             assert a in (64, 128) and 1 <= b <= 4
-        else:
+        elif op == B.TUNE_MIXED:
             assert 1 <= a <= 4 and 1 <= b <= 4
+        else:
+            assert a == 1 or (a == 2 and bucket >= 4)
 
 
 def test_override_semantics(h):
@@ -104,12 +118,17 @@ def test_override_semantics(h):
         assert B.set_tuning_override(h, B.TUNE_GEMM, 128, 0) == inv
         assert B.set_tuning_override(h, B.TUNE_GROUPED, 256, 1) == inv
         assert B.set_tuning_override(h, B.TUNE_MIXED, 5, 1) == inv
-        assert B.set_tuning_override(h, 3, 1, 1) == inv
+        assert B.set_tuning_override(h, 4, 1, 1) == inv
         assert B.set_tuning_override(None, 0, 64, 8) == status_code("HANDLE_IS_NULLPTR")
         assert B.set_tuning_override(h, B.TUNE_MIXED, 1, 2) == 0
         assert B.get_tuning_choice(h, B.TUNE_MIXED, 3) == (0, 1, 2)
+        # LU: two rows per lane only above n = 8
+        assert B.set_tuning_override(h, B.TUNE_LU, 3, 1) == inv
+        assert B.set_tuning_override(h, B.TUNE_LU, 2, 1) == 0
+        assert B.get_tuning_choice(h, B.TUNE_LU, _bucket(h, B.TUNE_LU, 32, 4096))[1] == 2
+        assert B.get_tuning_choice(h, B.TUNE_LU, _bucket(h, B.TUNE_LU, 8, 4096))[1] != 2
     finally:
-        for op in range(3):
+        for op in range(4):
             assert B.set_tuning_override(h, op, -1) == 0
     assert B.get_tuning_choice(h, B.TUNE_GEMM, f32_bucket) == before_f32
     assert B.get_tuning_choice(h, 0, 60)[0] == inv
@@ -120,7 +139,7 @@ def test_built_in_choices_without_table_rows(h):
     rows = {(op, b) for op, b, _, _ in _table_rows()}
     # buckets with no row fall back to a zero choice in the query (the calls use
     # their own built-in shape); buckets with a row report it
-    for op, count in ((0, 60), (1, 3), (2, 4)):
+    for op, count in ((0, 60), (1, 3), (2, 4), (3, 16)):
         for bucket in range(count):
             st, a, b = B.get_tuning_choice(h, op, bucket)
             assert st == 0

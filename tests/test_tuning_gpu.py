@@ -21,7 +21,7 @@ def h():
     handle = B.Handle()
     assert handle.set_stream(torch.cuda.current_stream().cuda_stream) == 0
     yield handle.value
-    for op in range(3):
+    for op in range(4):
         B.set_tuning_override(handle.value, op, -1)
     torch.cuda.synchronize()
     handle.destroy()
@@ -191,3 +191,25 @@ def test_direct_to_lds_kernel_matches_register_staged(h, kinds, trans, shape):
     pad_before = C0.cpu().reshape(n, ldc)[:, m:]
     pad_after = outs[0].cpu().reshape(n, ldc)[:, m:]
     assert torch.equal(pad_before, pad_after)
+
+
+@pytest.mark.parametrize("n", [9, 16, 24, 32, 40, 64])
+def test_lu_rows_per_lane_agree(h, n):
+    """One or two matrix rows per lane: identical factors, pivots and info (ragged batch, ties)."""
+    dev = torch.device("cuda")
+    batch = 37
+    rng = np.random.default_rng(n)
+    A0 = rng.integers(-2, 3, size=(batch, n, n)).astype(np.float32)    # many exact pivot ties
+    A0[5] = rng.standard_normal((n, n)).astype(np.float32)
+    A0[7][:, 3] = 0.0                                                   # zero pivot
+    dA0 = torch.from_numpy(A0).to(dev)
+    LU = dA0.clone()
+    piv = torch.empty((batch, n), dtype=torch.int32, device=dev)
+    info = torch.empty(batch, dtype=torch.int32, device=dev)
+    ptrs = B.ptr_array(LU)
+
+    def run():
+        LU.copy_(dA0)
+        piv.fill_(-7)
+        assert B.sgetrf_batched(h, n, ptrs.data_ptr(), n, piv.data_ptr(), info.data_ptr(), batch) == 0
+    _all_choices_equal(h, B.TUNE_LU, [(1, 1), (2, 1)], run, lambda: [LU, piv, info])
