@@ -270,7 +270,7 @@ class Tuner:
     def write_table(self, path, keep_from=None):
         """Writes the measured rows plus, from `keep_from`, the rows of operations not tuned in this run."""
         props = self.torch.cuda.get_device_properties(0)
-        dev_name = "%s %s" % (props.name, getattr(props, "gcnArchName", ""))
+        dev_name = "%s (%s)" % (props.name, getattr(props, "gcnArchName", "").split(":")[0])
         tuned = {op for op, _, _, _ in self.rows}
         if keep_from and os.path.exists(keep_from):
             for op, bucket, a, b in read_table(keep_from):
