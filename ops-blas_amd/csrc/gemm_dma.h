@@ -14,8 +14,8 @@
 //
 // LDS layouts (both conflict-free for their reads):
 //   K-major (operand stored k-contiguous): 256 rows x 64 bytes (32 k), 16-byte
-//     chunk q of row r at slot q ^ ((r >> 2) & 3); a fragment is one
-//     ds_read_b128 of 8 consecutive k.
+//     chunk q of row r at slot q ^ KmKey(r); a fragment is one ds_read_b128 of
+//     8 consecutive k (served 8 lanes = 128 bytes at a time).
 //   M-major (operand stored m- or n-contiguous): 32 rows (one k each) x 512
 //     bytes, chunk c of row kk at slot c ^ MmKey(kk); a fragment is two
 //     transposing reads (ds_read_b64_tr_b16) of 4 k each.
@@ -50,9 +50,13 @@ __device__ __forceinline__ int StageK(int s, int kk)
 {
     return 64 * (s >> 1) + 16 * (kk >> 3) + 8 * (s & 1) + (kk & 7);
 }
+// K-major swizzle key: the 16-byte slots of rows r..r+7 (and r..r+15) read at
+// one chunk land on distinct positions of a 128-byte (and 256-byte) bank
+// window.  It depends on r mod 16 only, so fragments 16 rows apart share it.
+__device__ __forceinline__ int KmKey(int r) { return ((r >> 1) & 3) ^ ((r >> 3) & 1); }
 __device__ __forceinline__ uint32_t KmOffset(int r, int q)
 {
-    return static_cast<uint32_t>(r * 64 + ((q ^ ((r >> 2) & 3)) << 4));
+    return static_cast<uint32_t>(r * 64 + ((q ^ KmKey(r)) << 4));
 }
 __device__ __forceinline__ int MmKey(int kk) { return (2 * (kk & 3)) ^ (8 * ((kk >> 3) & 1)); }
 __device__ __forceinline__ uint32_t MmOffset(int kk, int c)
@@ -86,7 +90,7 @@ struct DmaSource {
                 src[i] = X + static_cast<size_t>(k) * ld + r;
             } else {
                 const int r = min(row0 + 16 * j + (lane >> 2), rows - 1);
-                const int q = (lane & 3) ^ (((16 * j + (lane >> 2)) >> 2) & 3);
+                const int q = (lane & 3) ^ KmKey(16 * j + (lane >> 2));
                 src[i] = X + static_cast<size_t>(r) * ld + 16 * q;
             }
         }

@@ -27,11 +27,16 @@ __device__ __forceinline__ uint32_t LdsAddr(const void* p)
 // per lane) to LDS bytes [lds + 16 l, lds + 16 l + 16); lds is wave-uniform.
 // The LDS base goes through M0, which needs one wait state between its write
 // and the load that reads it.
+// (M0 is listed as clobbered; the compiler warns that it reserves M0, and no
+// code around these loads uses it.)
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
 __device__ __forceinline__ void LdsDma16(const void* g, uint32_t lds)
 {
     asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" : : "v"(g), "s"(lds)
                  : "memory", "m0");
 }
+#pragma clang diagnostic pop
 
 // Waits until at most N vector-memory operations of this wave are in flight
 // (they complete in issue order).
