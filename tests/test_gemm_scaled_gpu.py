@@ -48,6 +48,29 @@ def test_quantizer_matches_closed_form(handle, qt, count, scale):
     assert np.array_equal(q, ref), int((q != ref).sum())
 
 
+@pytest.mark.parametrize("xoff,qoff", [(1, 0), (0, 1), (3, 2)])
+def test_quantizer_unaligned_operands(handle, xoff, qoff):
+    """x not 16-byte or q not 4-byte aligned: the element-wise path, same codes."""
+    count = 70001
+    rng = np.random.default_rng(xoff * 10 + qoff)
+    x = (rng.standard_normal(count + xoff) * 7.0).astype(np.float32)
+    dx = to_dev(x)
+    q = torch.zeros(count + qoff, dtype=torch.uint8, device="cuda")
+    s = torch.zeros(1, dtype=torch.float32, device="cuda")
+    qt = "FP8_E5M2"
+    assert B.quantize_fp8(handle.value, count, dx.data_ptr() + 4 * xoff, B.DTYPES[qt], q.data_ptr() + qoff,
+                          s.data_ptr()) == 0
+    torch.cuda.synchronize()
+    xs = x[xoff:]
+    amax = np.float32(np.abs(xs).max())
+    fmax = np.float32(FMAX[qt])
+    assert float(s.cpu().numpy()[0]) == np.float32(amax / fmax)
+    ref = O.to_fp8((xs * np.float32(fmax / amax)).astype(np.float32), False)
+    got = q.cpu().numpy()
+    assert np.array_equal(got[qoff:], ref)
+    assert not got[:qoff].any()
+
+
 def test_quantizer_all_zero(handle):
     q, s, _, _ = _quantize(handle, np.zeros(4097, np.float32), "FP8_E4M3FN")
     assert s == 1.0 and not q.any()
