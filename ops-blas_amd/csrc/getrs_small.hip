@@ -7,7 +7,8 @@
 // unknown needs in a column-oriented substitution.  The right-hand sides of a
 // chunk (RC columns of B) sit in registers too, so each substitution step is
 // one broadcast of the solved value inside the segment (DPP row_newbcast for
-// S = 16, v_readlane otherwise) and one fmaf per lane and right-hand side: no
+// S = 16, plus v_permlane16_swap for S = 32, v_readlane for S = 64) and one
+// fmaf per lane and right-hand side: no
 // LDS, no barriers.  Row interchanges: every lane traces its final position
 // back through the swap list to find which row of B it loads (trans N) or
 // stores (trans T).  Element by element the operations and their order are
@@ -30,9 +31,12 @@ __device__ __forceinline__ float SegBroadcast(float v)
     if constexpr (S == 16) {
         return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + J, 0xF, 0xF, false));
     } else if constexpr (S == 32) {
-        const float lo = dev::ReadLane(v, J);
-        const float hi = dev::ReadLane(v, 32 + J);
-        return (threadIdx.x & 32) ? hi : lo;
+        // lane J % 16 of every 16-lane row, then the row holding J of each
+        // 32-lane segment to both of its rows (v_permlane16_swap)
+        const uint32_t t = static_cast<uint32_t>(
+            __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + (J & 15), 0xF, 0xF, false));
+        const auto p = __builtin_amdgcn_permlane16_swap(t, t, false, false);
+        return __uint_as_float(J < 16 ? p[0] : p[1]);
     } else {
         return dev::ReadLane(v, J);
     }
