@@ -267,5 +267,226 @@ __global__ __launch_bounds__(dma::kThreads) void GemmDmaKernel(GemmArgs p)
     StoreTileC<OK, TM, TN, false>(q, acc, m0 + wm, n0 + wn, lane);
 }
 
+// ---------------------------------------------------------------------------
+// FP8 operands on the native v_mfma_scale_f32_16x16x128_f8f6f4 (the scaled
+// entry).  Same tile and waves; stages of 128 k (one MFMA per fragment pair),
+// two LDS stages of 64 KiB, one stage in flight while the other is consumed.
+//   K-major: 256 rows x 128 bytes, the register-staged kernel's swizzle
+//     (SwizzledOffset); a fragment is two ds_read_b128 of 16 consecutive k.
+//   M-major: 128 rows (one k each) x 256 bytes, 16-byte chunk c of row kk at
+//     slot c ^ Mm8Key(kk); a fragment is four ds_read_b64_tr_b8 of 8 k each.
+// The MFMA of stage s sums k = 128 s + 32 (lane / 16) + 0..31, as the
+// register-staged kernel's does for its k-block s: bit-identical results.
+// ---------------------------------------------------------------------------
+namespace dma8 {
+
+constexpr int kKS = 128;
+constexpr int kStages = 2;
+constexpr int kOpBytes = 256 * kKS;  // 32 KiB per operand and stage
+
+__device__ __forceinline__ int Mm8Key(int kk) { return (kk & 7) ^ (8 * ((kk >> 5) & 1)); }
+__device__ __forceinline__ uint32_t Mm8Offset(int kk, int c)
+{
+    return static_cast<uint32_t>(kk * 256 + ((c ^ Mm8Key(kk)) << 4));
+}
+
+// Per-lane DMA sources of one operand for its four 1-KiB instructions per
+// wave and stage (wave w issues w, w + 8, w + 16, w + 24), as byte pointers at
+// stage 0, and the byte step from one stage to the next.
+struct DmaSource8 {
+    const uint8_t* src[4];
+    long long step;
+
+    template <bool MM>
+    __device__ __forceinline__ void Setup(const uint8_t* X, int ld, int row0, int rows, int wave, int lane)
+    {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int j = wave + dma::kWaves * i;
+            if (MM) {
+                const int kk = 4 * j + (lane >> 4);
+                const int c = (lane & 15) ^ Mm8Key(kk);
+                const int last = ((rows + 15) & ~15) - 16;  // last 16-aligned chunk inside the column
+                const int r = min(row0 + 16 * c, last);
+                src[i] = X + static_cast<size_t>(kk) * ld + r;
+            } else {
+                const int rl = 8 * j + (lane >> 3);
+                const int r = min(row0 + rl, rows - 1);
+                const int c = (lane & 7) ^ ((rl ^ (rl >> 3)) & 7);
+                src[i] = X + static_cast<size_t>(r) * ld + 16 * c;
+            }
+        }
+        step = MM ? static_cast<long long>(kKS) * ld : kKS;
+    }
+
+    __device__ __forceinline__ void Issue(int s, uint8_t* buf, int wave) const
+    {
+        const long long off = static_cast<long long>(s) * step;
+        const uint32_t base = dev::LdsAddr(buf);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            dev::LdsDma16(src[i] + off, __builtin_amdgcn_readfirstlane(base + 1024u * (wave + dma::kWaves * i)));
+        }
+    }
+};
+
+// Register-path load of a partial stage (k tail), bounds-checked, zero fill.
+template <bool MM>
+__device__ __forceinline__ void FillStage8(uint8_t* buf, const uint8_t* X, int ld, int row0, int rows, int s, int K,
+                                           int t)
+{
+    for (int e = t; e < 256 * kKS; e += dma::kThreads) {
+        const int r = MM ? (e & 255) : (e >> 7);
+        const int kk = MM ? (e >> 8) : (e & 127);
+        const int k = kKS * s + kk;
+        uint8_t v = 0;
+        if (row0 + r < rows && k < K) {
+            v = MM ? X[static_cast<size_t>(k) * ld + row0 + r] : X[static_cast<size_t>(row0 + r) * ld + k];
+        }
+        const uint32_t off = MM ? Mm8Offset(kk, r >> 4) + (r & 15) : SwizzledOffset(r, kk);
+        buf[off] = v;
+    }
+}
+
+// Byte offset base of a lane's transposing reads in an M-major stage for the
+// wave's fragments starting at tile index wbase: fragment f (16 indices, f
+// below the alignment of wbase / 16) reads 8 k at (TrBase8 ^ (f << 4)) +
+// 2048 t, t = 0..3.
+__device__ __forceinline__ uint32_t TrBase8(int wbase, int lane)
+{
+    const int g = lane >> 4;
+    const int j = lane & 15;
+    const int key = (j >> 1) ^ (8 * (g & 1));
+    return static_cast<uint32_t>((32 * g + (j >> 1)) * 256 + 8 * (j & 1) + (((wbase >> 4) ^ key) << 4));
+}
+
+}  // namespace dma8
+
+// 256 x 256 tile, FP8 operands (KA, KB in kE4M3S / kE5M2S), direct-to-LDS
+// loads; AMM / BMM as for GemmDmaKernel.  scaleA * scaleB * alpha in the
+// epilogue.
+template <int KA, int KB, int OK, bool AMM, bool BMM>
+__global__ __launch_bounds__(dma::kThreads) void GemmDmaFp8Kernel(GemmArgs p)
+{
+    using namespace dma8;
+    __shared__ __attribute__((aligned(1024))) uint8_t lds[kStages][2][kOpBytes];
+
+    int bid = blockIdx.x;
+    if (gridDim.x % 8 == 0) {
+        bid = (bid % 8) * (gridDim.x / 8) + bid / 8;
+    }
+    int tm, tn;
+    TileOf(bid, p.tilesM, p.tilesN, p.group, tm, tn);
+    const int m0 = tm * dma::kBM;
+    const int n0 = tn * dma::kBN;
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int wm = (wave % dma::kWGM) * (dma::kBM / dma::kWGM);
+    const int wn = (wave / dma::kWGM) * (dma::kBN / dma::kWGN);
+    const uint8_t* A = static_cast<const uint8_t*>(p.A);
+    const uint8_t* B = static_cast<const uint8_t*>(p.B);
+    const int K = p.k;
+    const int nFull = K / kKS;
+    const int nst = (K + kKS - 1) / kKS;
+
+    DmaSource8 da;
+    DmaSource8 db;
+    da.Setup<AMM>(A, p.lda, m0, p.m, wave, lane);
+    db.Setup<BMM>(B, p.ldb, n0, p.n, wave, lane);
+
+    auto issue = [&](int s) {
+        uint8_t* bufA = lds[s % kStages][0];
+        uint8_t* bufB = lds[s % kStages][1];
+        if (s < nFull) {
+            da.Issue(s, bufA, wave);
+            db.Issue(s, bufB, wave);
+        } else {
+            FillStage8<AMM>(bufA, A, p.lda, m0, p.m, s, K, t);
+            FillStage8<BMM>(bufB, B, p.ldb, n0, p.n, s, K, t);
+        }
+    };
+
+    const int fr = lane & 15;
+    const int fq = (lane >> 4) * 32;
+    // K-major fragment offsets for row groups i % 4 (the swizzle key repeats every 64 rows)
+    uint32_t kmA[4], kmB[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        kmA[i] = SwizzledOffset(wm + i * 16 + fr, fq);
+        kmB[i] = SwizzledOffset(wn + i * 16 + fr, fq);
+    }
+    const uint32_t trA = TrBase8(wm, lane);
+    const uint32_t trB = TrBase8(wn, lane);
+
+    constexpr int TM = dma::kBM / dma::kWGM / 16;  // 8
+    constexpr int TN = dma::kBN / dma::kWGN / 16;  // 4
+    f32x4 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    auto fragment = [&](const uint8_t* buf, bool mm, uint32_t km, uint32_t tr, int f, uint4 (&out)[2]) {
+        if (mm) {
+            const uint32_t o = tr ^ (static_cast<uint32_t>(f) << 4);
+            const uint2 r0 = dev::ReadTr8(buf + o);
+            const uint2 r1 = dev::ReadTr8(buf + o + 2048);
+            const uint2 r2 = dev::ReadTr8(buf + o + 4096);
+            const uint2 r3 = dev::ReadTr8(buf + o + 6144);
+            out[0] = make_uint4(r0.x, r0.y, r1.x, r1.y);
+            out[1] = make_uint4(r2.x, r2.y, r3.x, r3.y);
+        } else {
+            out[0] = *reinterpret_cast<const uint4*>(buf + km);
+            out[1] = *reinterpret_cast<const uint4*>(buf + (km ^ 16u));
+        }
+    };
+
+    if (nst > 0) {
+        issue(0);
+    }
+    for (int s = 0; s < nst; ++s) {
+        // this wave's DMA for stage s is the only one in flight
+        dev::WaitVmcnt<0>();
+        dev::BarrierLds();
+        dev::SetPriority<1>();
+        const uint8_t* a = lds[s % kStages][0];
+        const uint8_t* b = lds[s % kStages][1];
+        uint4 fb[TN][2];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            fragment(b, BMM, kmB[j % 4] + (j / 4) * 8192u, trB, j, fb[j]);
+        }
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+            uint4 fa[2];
+            fragment(a, AMM, kmA[i % 4] + (i / 4) * 8192u, trA, i, fa);
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                Mfma<KA, KB>::Run(fa, fb[j], acc[i][j]);
+            }
+            // the next stage's loads go out once the MFMA pipe is fed
+            if (i == 1 && s + 1 < nst) {
+                issue(s + 1);
+            }
+        }
+        dev::SetPriority<0>();
+    }
+
+    TileJob q;
+    q.C = p.C;
+    q.m = p.m;
+    q.n = p.n;
+    q.k = p.k;
+    q.ldc = p.ldc;
+    q.vecC = p.vecC != 0;
+    float alpha = p.alpha;
+    if (p.scaleA != nullptr) alpha *= *p.scaleA;
+    if (p.scaleB != nullptr) alpha *= *p.scaleB;
+    q.alpha = alpha;
+    q.beta = p.beta;
+    StoreTileC<OK, TM, TN, false>(q, acc, m0 + wm, n0 + wn, lane);
+}
+
 }  // namespace gemm
 }  // namespace aclblas

@@ -895,6 +895,25 @@ inline void LaunchGemmTyped(hipStream_t st, GemmArgs p, int batch, int tile, int
             return;
         }
     }
+    if constexpr (KA == kE4M3S || KA == kE5M2S) {
+        // direct-to-LDS FP8 kernel: one problem, 16-byte aligned operands
+        if (tile == 256 && p.Aarr == nullptr && p.vecA != 0 && p.vecB != 0) {
+            p.tilesM = (p.m + 255) / 256;
+            p.tilesN = (p.n + 255) / 256;
+            const dim3 grid(p.tilesM * p.tilesN);
+            const dim3 block(dma::kThreads);
+            if (p.transA == 0 && p.transB == 0) {
+                hipLaunchKernelGGL((GemmDmaFp8Kernel<KA, KB, OK, true, false>), grid, block, 0, st, p);
+            } else if (p.transA == 0) {
+                hipLaunchKernelGGL((GemmDmaFp8Kernel<KA, KB, OK, true, true>), grid, block, 0, st, p);
+            } else if (p.transB == 0) {
+                hipLaunchKernelGGL((GemmDmaFp8Kernel<KA, KB, OK, false, false>), grid, block, 0, st, p);
+            } else {
+                hipLaunchKernelGGL((GemmDmaFp8Kernel<KA, KB, OK, false, true>), grid, block, 0, st, p);
+            }
+            return;
+        }
+    }
     if constexpr (EB <= 2) {
         if (tile == 256) {
             p.tilesM = (p.m + 255) / 256;

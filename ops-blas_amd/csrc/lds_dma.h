@@ -72,5 +72,18 @@ __device__ __forceinline__ uint2 ReadTr16(const void* lds)
     return __builtin_bit_cast(uint2, v);
 }
 
+typedef int V2i32 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) V2i32 LdsV2i32;
+
+// ds_read_b64_tr_b8.  Within each group of 16 lanes, let M[j][e] (e < 8) be
+// the eight bytes at lane j's address.  Lane i receives M[2 e + i / 8][i % 8]
+// for e = 0..7: with lane j = 2 e + a pointing at bytes (row 8a .. 8a+7,
+// column e) of a row-contiguous block, lane i gets row i, columns 0..7.
+__device__ __forceinline__ uint2 ReadTr8(const void* lds)
+{
+    const V2i32 v = __builtin_amdgcn_ds_read_tr8_b64_v2i32((LdsV2i32*)(lds));
+    return __builtin_bit_cast(uint2, v);
+}
+
 }  // namespace dev
 }  // namespace aclblas

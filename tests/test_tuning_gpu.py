@@ -213,3 +213,39 @@ def test_lu_rows_per_lane_agree(h, n):
         piv.fill_(-7)
         assert B.sgetrf_batched(h, n, ptrs.data_ptr(), n, piv.data_ptr(), info.data_ptr(), batch) == 0
     _all_choices_equal(h, B.TUNE_LU, [(1, 1), (2, 1)], run, lambda: [LU, piv, info])
+
+
+@pytest.mark.parametrize("kinds", [("FP8_E4M3FN", "FP8_E4M3FN", "FP16"), ("FP8_E5M2", "FP8_E4M3FN", "FP32"),
+                                   ("FP8_E4M3FN", "FP8_E5M2", "BF16")])
+@pytest.mark.parametrize("trans", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("shape", [(512, 384, 640), (264, 520, 300), (40, 33, 100)])
+def test_direct_to_lds_fp8_kernel_matches_register_staged(h, kinds, trans, shape):
+    """aclblasGemmExScaled with 256 tiles and 16-byte aligned operands runs the FP8 direct-to-LDS kernel,
+    with 128 tiles the register-staged kernel: the same MFMA sums, bit-identical outputs."""
+    m, n, k = shape
+    at, bt, ct = kinds
+    ta, tb = trans
+    dev = torch.device("cuda")
+    rowsA, colsA = (k, m) if ta else (m, k)
+    rowsB, colsB = (n, k) if tb else (k, n)
+    r16 = lambda x: (x + 15) // 16 * 16
+    lda, ldb, ldc = r16(rowsA), r16(rowsB), m + 5
+    rng = np.random.default_rng(m + 3 * n + 7 * k + 11 * ta + 13 * tb)
+    codes = lambda cnt: (rng.integers(0, 0x40, cnt, dtype=np.uint8) | (rng.integers(0, 2, cnt, dtype=np.uint8) << 7))
+    A = torch.from_numpy(codes(lda * colsA)).to(dev)
+    Bm = torch.from_numpy(codes(ldb * colsB)).to(dev)
+    tdt = {"FP16": torch.float16, "BF16": torch.bfloat16, "FP32": torch.float32}[ct]
+    C0 = torch.randn(ldc * n, generator=torch.Generator(device="cpu").manual_seed(k)).to(tdt).to(dev)
+    scale = torch.tensor([0.5, 0.25], device=dev)
+    outs = []
+    for tile in (256, 128):
+        assert B.set_tuning_override(h, B.TUNE_GEMM, tile, 8) == 0
+        C = C0.clone()
+        assert B.gemm_ex_scaled(h, B.OP_T if ta else B.OP_N, B.OP_T if tb else B.OP_N, m, n, k, 1.25, A.data_ptr(),
+                                B.DTYPES[at], lda, scale.data_ptr(), Bm.data_ptr(), B.DTYPES[bt], ldb,
+                                scale.data_ptr() + 4, -0.5, C.data_ptr(), B.DTYPES[ct], ldc) == 0
+        torch.cuda.synchronize()
+        outs.append(C)
+    assert torch.equal(outs[0], outs[1]), "direct-to-LDS and register-staged FP8 kernels differ"
+    pad_before = C0.cpu().reshape(n, ldc)[:, m:]
+    assert torch.equal(pad_before, outs[0].cpu().reshape(n, ldc)[:, m:])
