@@ -278,7 +278,7 @@ class Tuner:
                     self.rows.append((op, bucket, a, b))
         with open(path, "w") as f:
             f.write("// Launch-choice table {op, bucket, a, b}: written by ops-blas_amd/autotune.py\n")
-            f.write("// from timings on %s (%d CUs), %s.\n" % (dev_name, self.cus,
+            f.write("// from timings on %s, %d CUs, %s.\n" % (dev_name, self.cus,
                                                              datetime.date.today().isoformat()))
             for op, bucket, a, b in sorted(self.rows):
                 f.write("{%d, %d, %d, %d},\n" % (op, bucket, a, b))
