@@ -488,5 +488,211 @@ __global__ __launch_bounds__(dma::kThreads) void GemmDmaFp8Kernel(GemmArgs p)
     StoreTileC<OK, TM, TN, false>(q, acc, m0 + wm, n0 + wn, lane);
 }
 
+// ---------------------------------------------------------------------------
+// 16-bit operands with 64-k stages: two LDS stages of 64 KiB, one barrier per
+// 64 k, one stage in flight (the FP8 kernel's pipeline).
+//   K-major: 256 rows x 128 bytes (64 k), the register-staged kernel's
+//     swizzle (SwizzledOffset); a fragment half is one ds_read_b128.
+//   M-major: 64 rows (one k each) x 512 bytes, chunk c of row kk at slot
+//     c ^ Mm2Key(kk); a fragment half is two ds_read_b64_tr_b16.
+// Per 64 k each accumulator takes the MFMA of k = 16 (lane / 16) + 0..7 and
+// then that of + 8..15, as in the register-staged kernel: bit-identical.
+// ---------------------------------------------------------------------------
+namespace dma2 {
+
+constexpr int kKS = 64;
+constexpr int kStages = 2;
+constexpr int kOpBytes = 256 * 128;  // 32 KiB per operand and stage
+
+__device__ __forceinline__ int Mm2Key(int kk) { return (2 * (kk & 3)) ^ (8 * ((kk >> 4) & 1)); }
+__device__ __forceinline__ uint32_t Mm2Offset(int kk, int c)
+{
+    return static_cast<uint32_t>(kk * 512 + ((c ^ Mm2Key(kk)) << 4));
+}
+
+// DMA sources (byte pointers at stage 0) of this lane's four 1-KiB
+// instructions per operand and stage, and the byte step per stage.
+struct DmaSource2 {
+    const uint8_t* src[4];
+    long long step;
+
+    template <bool MM>
+    __device__ __forceinline__ void Setup(const uint16_t* X, int ld, int row0, int rows, int wave, int lane)
+    {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int j = wave + dma::kWaves * i;
+            if (MM) {
+                const int kk = 2 * j + (lane >> 5);
+                const int c = (lane & 31) ^ Mm2Key(kk);
+                const int last = ((rows + 7) & ~7) - 8;
+                const int r = min(row0 + 8 * c, last);
+                src[i] = reinterpret_cast<const uint8_t*>(X + static_cast<size_t>(kk) * ld + r);
+            } else {
+                const int rl = 8 * j + (lane >> 3);
+                const int r = min(row0 + rl, rows - 1);
+                const int c = (lane & 7) ^ ((rl ^ (rl >> 3)) & 7);
+                src[i] = reinterpret_cast<const uint8_t*>(X + static_cast<size_t>(r) * ld) + 16 * c;
+            }
+        }
+        step = MM ? 2ll * kKS * ld : 2ll * kKS;
+    }
+
+    __device__ __forceinline__ void Issue(int s, uint8_t* buf, int wave) const
+    {
+        const long long off = static_cast<long long>(s) * step;
+        const uint32_t base = dev::LdsAddr(buf);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            dev::LdsDma16(src[i] + off, __builtin_amdgcn_readfirstlane(base + 1024u * (wave + dma::kWaves * i)));
+        }
+    }
+};
+
+template <bool MM>
+__device__ __forceinline__ void FillStage2(uint8_t* buf, const uint16_t* X, int ld, int row0, int rows, int s, int K,
+                                           int t)
+{
+    for (int e = t; e < 256 * kKS; e += dma::kThreads) {
+        const int r = MM ? (e & 255) : (e >> 6);
+        const int kk = MM ? (e >> 8) : (e & 63);
+        const int k = kKS * s + kk;
+        uint16_t v = 0;
+        if (row0 + r < rows && k < K) {
+            v = MM ? X[static_cast<size_t>(k) * ld + row0 + r] : X[static_cast<size_t>(row0 + r) * ld + k];
+        }
+        const uint32_t off = MM ? Mm2Offset(kk, r >> 3) + 2u * (r & 7) : SwizzledOffset(r, 2 * kk);
+        *reinterpret_cast<uint16_t*>(buf + off) = v;
+    }
+}
+
+// Transposing-read base: fragment f reads its half h at
+// ((Tr2Base ^ (f << 5)) + 4096 h) and the second 4 k at + 2048.
+__device__ __forceinline__ uint32_t Tr2Base(int wbase, int lane)
+{
+    const int g = lane >> 4;
+    const int j = lane & 15;
+    const int key = ((j >> 2) & 3) ^ (4 * (g & 1));
+    return static_cast<uint32_t>((16 * g + (j >> 2)) * 512 + (((wbase >> 4) ^ key) << 5) + (j & 3) * 8);
+}
+
+}  // namespace dma2
+
+template <int KA, int OK, bool AMM, bool BMM>
+__global__ __launch_bounds__(dma::kThreads) void GemmDma64Kernel(GemmArgs p)
+{
+    using namespace dma2;
+    __shared__ __attribute__((aligned(1024))) uint8_t lds[kStages][2][kOpBytes];
+
+    int bid = blockIdx.x;
+    if (gridDim.x % 8 == 0) {
+        bid = (bid % 8) * (gridDim.x / 8) + bid / 8;
+    }
+    int tm, tn;
+    TileOf(bid, p.tilesM, p.tilesN, p.group, tm, tn);
+    const int m0 = tm * dma::kBM;
+    const int n0 = tn * dma::kBN;
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int wm = (wave % dma::kWGM) * (dma::kBM / dma::kWGM);
+    const int wn = (wave / dma::kWGM) * (dma::kBN / dma::kWGN);
+    const uint16_t* A = static_cast<const uint16_t*>(p.A);
+    const uint16_t* B = static_cast<const uint16_t*>(p.B);
+    const int K = p.k;
+    const int nFull = K / kKS;
+    const int nst = (K + kKS - 1) / kKS;
+
+    DmaSource2 da;
+    DmaSource2 db;
+    da.Setup<AMM>(A, p.lda, m0, p.m, wave, lane);
+    db.Setup<BMM>(B, p.ldb, n0, p.n, wave, lane);
+
+    auto issue = [&](int s) {
+        uint8_t* bufA = lds[s % kStages][0];
+        uint8_t* bufB = lds[s % kStages][1];
+        if (s < nFull) {
+            da.Issue(s, bufA, wave);
+            db.Issue(s, bufB, wave);
+        } else {
+            FillStage2<AMM>(bufA, A, p.lda, m0, p.m, s, K, t);
+            FillStage2<BMM>(bufB, B, p.ldb, n0, p.n, s, K, t);
+        }
+    };
+
+    const int fr = lane & 15;
+    const int fq = (lane >> 4) * 32;
+    uint32_t kmA[4], kmB[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        kmA[i] = SwizzledOffset(wm + i * 16 + fr, fq);
+        kmB[i] = SwizzledOffset(wn + i * 16 + fr, fq);
+    }
+    const uint32_t trA = Tr2Base(wm, lane);
+    const uint32_t trB = Tr2Base(wn, lane);
+
+    constexpr int TM = dma::kBM / dma::kWGM / 16;  // 8
+    constexpr int TN = dma::kBN / dma::kWGN / 16;  // 4
+    f32x4 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    auto half = [&](const uint8_t* buf, bool mm, uint32_t km, uint32_t tr, int f, int h) -> uint4 {
+        if (mm) {
+            const uint32_t o = (tr ^ (static_cast<uint32_t>(f) << 5)) + 4096u * h;
+            const uint2 lo = dev::ReadTr16(buf + o);
+            const uint2 hi = dev::ReadTr16(buf + o + 2048);
+            return make_uint4(lo.x, lo.y, hi.x, hi.y);
+        }
+        return *reinterpret_cast<const uint4*>(buf + (km ^ (16u * h)));
+    };
+
+    if (nst > 0) {
+        issue(0);
+    }
+    for (int s = 0; s < nst; ++s) {
+        // this wave's DMA for stage s is the only one in flight
+        dev::WaitVmcnt<0>();
+        dev::BarrierLds();
+        dev::SetPriority<1>();
+        const uint8_t* a = lds[s % kStages][0];
+        const uint8_t* b = lds[s % kStages][1];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            uint4 fb[TN];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                fb[j] = half(b, BMM, kmB[j % 4] + (j / 4) * 8192u, trB, j, h);
+            }
+#pragma unroll
+            for (int i = 0; i < TM; ++i) {
+                const uint4 fa = half(a, AMM, kmA[i % 4] + (i / 4) * 8192u, trA, i, h);
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    Mfma<KA, KA>::RunHalf(fa, fb[j], acc[i][j]);
+                }
+                // the next stage's loads go out once the MFMA pipe is fed
+                if (h == 0 && i == 1 && s + 1 < nst) {
+                    issue(s + 1);
+                }
+            }
+        }
+        dev::SetPriority<0>();
+    }
+
+    TileJob q;
+    q.C = p.C;
+    q.m = p.m;
+    q.n = p.n;
+    q.k = p.k;
+    q.ldc = p.ldc;
+    q.vecC = p.vecC != 0;
+    q.alpha = p.alpha;
+    q.beta = p.beta;
+    StoreTileC<OK, TM, TN, false>(q, acc, m0 + wm, n0 + wn, lane);
+}
+
 }  // namespace gemm
 }  // namespace aclblas

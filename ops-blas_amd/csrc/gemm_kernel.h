@@ -883,14 +883,16 @@ inline void LaunchGemmTyped(hipStream_t st, GemmArgs p, int batch, int tile, int
             p.tilesN = (p.n + 255) / 256;
             const dim3 grid(p.tilesM * p.tilesN);
             const dim3 block(dma::kThreads);
+            // m-contiguous A: 32-k stages (transposing reads, conflict-free);
+            // k-contiguous A: 64-k stages (128-byte rows, conflict-free)
             if (p.transA == 0 && p.transB == 0) {
                 hipLaunchKernelGGL((GemmDmaKernel<KA, OK, true, false>), grid, block, 0, st, p);
             } else if (p.transA == 0) {
                 hipLaunchKernelGGL((GemmDmaKernel<KA, OK, true, true>), grid, block, 0, st, p);
             } else if (p.transB == 0) {
-                hipLaunchKernelGGL((GemmDmaKernel<KA, OK, false, false>), grid, block, 0, st, p);
+                hipLaunchKernelGGL((GemmDma64Kernel<KA, OK, false, false>), grid, block, 0, st, p);
             } else {
-                hipLaunchKernelGGL((GemmDmaKernel<KA, OK, false, true>), grid, block, 0, st, p);
+                hipLaunchKernelGGL((GemmDma64Kernel<KA, OK, false, true>), grid, block, 0, st, p);
             }
             return;
         }
