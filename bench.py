@@ -62,6 +62,9 @@ def parse():
     ap.add_argument("--primary-only", action="store_true", help="skip the secondary workloads")
     ap.add_argument("--profile-gemm", action="store_true",
                     help="only run three BF16 x BF16 -> FP32 8192^3 GEMMs (for counter collection) and exit")
+    ap.add_argument("--profile-fp8", action="store_true",
+                    help="only quantise two 8192^2 FP32 operands to E4M3 and run three scaled FP8 8192^3 GEMMs "
+                         "(BF16 C) and exit")
     ap.add_argument("--profile-mixed", action="store_true",
                     help="only run the mixed-order solve of 10000 systems three times (for profiling) and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -353,6 +356,28 @@ def main():
         res = bench_mixed(torch, B, W, h, Timer(torch, stream), plain=False)
         h.destroy()
         print(json.dumps(res))
+        return
+    if args.profile_fp8:
+        h = B.Handle()
+        h.set_stream(torch.cuda.current_stream().cuda_stream)
+        m = 8192
+        e4 = B.ACL_FLOAT8_E4M3FN
+        qa = torch.empty(m * m, dtype=torch.uint8, device=dev)
+        qb = torch.empty(m * m, dtype=torch.uint8, device=dev)
+        sc = torch.empty(2, device=dev)
+        cb = torch.empty(m * m, dtype=torch.bfloat16, device=dev)
+        for q, seed in ((qa, 1), (qb, 2)):
+            x = torch.randn(m * m, device=dev, generator=torch.Generator(device=dev).manual_seed(seed))
+            assert B.quantize_fp8(h.value, m * m, x.data_ptr(), e4, q.data_ptr(),
+                                  sc.data_ptr() + (0 if q is qa else 4)) == 0
+            del x
+        for _ in range(3):
+            st = B.gemm_ex_scaled(h.value, B.OP_N, B.OP_N, m, m, m, 1.0, qa.data_ptr(), e4, m, sc.data_ptr(),
+                                  qb.data_ptr(), e4, m, sc.data_ptr() + 4, 0.0, cb.data_ptr(), B.ACL_BF16, m)
+            assert st == 0, st
+        torch.cuda.synchronize()
+        h.destroy()
+        print("fp8 profile run done")
         return
     if args.profile_gemm:
         h = B.Handle()
