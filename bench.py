@@ -67,6 +67,8 @@ def parse():
                          "(BF16 C) and exit")
     ap.add_argument("--profile-mixed", action="store_true",
                     help="only run the mixed-order solve of 10000 systems three times (for profiling) and exit")
+    ap.add_argument("--profile-matinv", action="store_true",
+                    help="only run the 4096 x 32 batched inverse (21 calls, for profiling) and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="before warm-up, run 5 elementwise negations of a MIB-MiB FP32 tensor (read and write "
                          "MIB MiB each): the known-size reference that scales hardware byte counters")
@@ -336,6 +338,29 @@ def bench_adjoint(torch, B, W, h, timer):
             "forward_solves_per_s": batch / (fwd_ms * 1e-3)}
 
 
+def bench_matinv(torch, B, W, h, timer):
+    """4096 inverses of order 32 (aclblasSmatinvBatched): stage + LU + inverse launches."""
+    dev = torch.device("cuda")
+    batch, n = 4096, 32
+    A, _ = W.lu_batch(batch, n, 1, seed=3, shift=4.0)
+    dA = torch.from_numpy(A).to(dev)
+    C = torch.empty_like(dA)
+    info = torch.zeros(batch, dtype=torch.int32, device=dev)
+    pA, pC = B.ptr_array(dA), B.ptr_array(C)
+
+    def inv():
+        st = B.smatinv_batched(h.value, n, pA.data_ptr(), n, pC.data_ptr(), n, info.data_ptr(), batch)
+        assert st == 0, st
+
+    inv()
+    ms = timer(inv, reps=20)
+    assert int(info.abs().sum().item()) == 0
+    # algorithmic bytes: A read, Ainv written, info, two pointers per matrix
+    per = 2 * n * n * 4 + 4 + 16
+    return {"batch": batch, "n": n, "ms": ms, "inverses_per_s": batch / (ms * 1e-3),
+            "hbm_frac_of_algorithmic_bytes": per * batch / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -354,6 +379,14 @@ def main():
         stream = torch.cuda.current_stream()
         h.set_stream(stream.cuda_stream)
         res = bench_mixed(torch, B, W, h, Timer(torch, stream), plain=False)
+        h.destroy()
+        print(json.dumps(res))
+        return
+    if args.profile_matinv:
+        h = B.Handle()
+        stream = torch.cuda.current_stream()
+        h.set_stream(stream.cuda_stream)
+        res = bench_matinv(torch, B, W, h, Timer(torch, stream))
         h.destroy()
         print(json.dumps(res))
         return
@@ -478,6 +511,7 @@ def main():
         sec["mixed_solve"] = bench_mixed(torch, B, W, h, timer)
         sec["gemm"] = bench_gemm(torch, B, W, h, timer)
         sec["adjoint"] = bench_adjoint(torch, B, W, h, timer)
+        sec["matinv"] = bench_matinv(torch, B, W, h, timer)
         out["secondary"] = sec
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, Bv)

@@ -182,6 +182,17 @@ aclblasStatus_t aclblasSgetriBatched(
     aclblasHandle_t handle, int n, const float* const Aarray[], int lda, const int* PivotArray,
     float* const Carray[], int ldc, int* infoArray, int batchSize);
 
+/* reference include/cann_ops_blas.h:434-436.  Ainv[i] = A[i]^-1 for
+ * 0 <= n <= 32 (pivoted LU, then the out-of-place inverse; A and Ainv are
+ * device arrays of device pointers, column-major, Ainv[i] must not overlap
+ * A[i]).  info (device, batchSize entries): the inverse's rule, j+1 for the
+ * zero u_jj with the largest j, 0 otherwise.  Uses batchSize * 8 +
+ * n^2 * batchSize * 4 + n * batchSize * 4 bytes of the handle's workspace and
+ * returns EXECUTION_FAILED when the workspace is smaller (it is not grown). */
+aclblasStatus_t aclblasSmatinvBatched(
+    aclblasHandle_t handle, int n, const float* const A[], int lda, float* const Ainv[], int lda_inv, int* info,
+    int batchSize);
+
 /* ------------------------------------------------------------------------ */
 /* Differentiation                                                           */
 /* ------------------------------------------------------------------------ */

@@ -60,6 +60,7 @@ _SIGNATURES = {
     "aclblasSgetrfBatched": [_P, _I, _P, _I, _P, _P, _I],
     "aclblasSgetrsBatched": [_P, _I, _I, _I, _P, _I, _P, _P, _I, _P, _I],
     "aclblasSgetriBatched": [_P, _I, _P, _I, _P, _P, _I, _P, _I],
+    "aclblasSmatinvBatched": [_P, _I, _P, _I, _P, _I, _P, _I],
     "aclblasSgetrsBatchedAdjoint": [_P, _I, _I, _P, _I, _P, _P, _I, _P, _I, _P, _I, _I],
     "aclblasSgemmGroupedBatched": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "aclblasSgesvBatchedMixed": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, _I, _P, _P, _P],
@@ -171,6 +172,10 @@ def sgetrs_batched(h, trans, n, nrhs, Aarray, lda, piv, Barray, ldb, info_host, 
 
 def sgetri_batched(h, n, Aarray, lda, piv, Carray, ldc, info, batch):
     return load().aclblasSgetriBatched(h, n, Aarray, lda, piv, Carray, ldc, info, batch)
+
+
+def smatinv_batched(h, n, Aarray, lda, Ainv, lda_inv, info, batch):
+    return load().aclblasSmatinvBatched(h, n, Aarray, lda, Ainv, lda_inv, info, batch)
 
 
 def _host_array(ctype, values):

@@ -19,6 +19,12 @@ aclblasStatus_t GetrfBatchedLaunch(const _aclblas_handle* h, int n, float* const
 aclblasStatus_t GetrsBatchedLaunch(hipStream_t st, bool trans, int n, int nrhs, const float* const* A, int lda,
                                    const int* piv, float* const* B, int ldb, int batch);
 
+// C = A^-1 from LU factors (out of place; C must not overlap A), info[b] = j+1
+// for the zero u_jj with the largest j, getrs.hip.  NOT_SUPPORTED when n is too
+// large for one LDS chunk.
+aclblasStatus_t GetriBatchedLaunch(hipStream_t st, int n, const float* const* A, int lda, const int* piv,
+                                   float* const* C, int ldc, int* info, int batch);
+
 // C[i] = alpha op(A[i]) op(B[i]) + beta C[i], FP32, device pointer arrays, on the
 // handle's stream with its tuned tile choice, gemm.hip.
 aclblasStatus_t SgemmBatchedLaunch(const _aclblas_handle* h, bool transA, bool transB, int m, int n, int k,

@@ -441,6 +441,20 @@ aclblasStatus_t GetrsBatchedLaunch(hipStream_t st, bool trans, int n, int nrhs, 
     return LaunchStatus("aclblasSgetrsBatched");
 }
 
+aclblasStatus_t GetriBatchedLaunch(hipStream_t st, int n, const float* const* A, int lda, const int* piv,
+                                   float* const* C, int ldc, int* info, int batch)
+{
+    if (LaunchGetriSmall(st, n, A, lda, piv, C, ldc, info, batch)) {
+        return LaunchStatus("aclblasSgetriBatched");
+    }
+    const int rc = ChooseRc(n, n);
+    if (rc <= 0) {
+        return ACLBLAS_STATUS_NOT_SUPPORTED;
+    }
+    LaunchSolve<kModeInv>(st, RoundRc(rc), A, n, lda, piv, C, ldc, n, info, batch);
+    return LaunchStatus("aclblasSgetriBatched");
+}
+
 }  // namespace aclblas
 
 extern "C" aclblasStatus_t aclblasSgetrsBatched(
@@ -483,15 +497,12 @@ extern "C" aclblasStatus_t aclblasSgetriBatched(
     if (n == 0 || batchSize == 0) {
         return ACLBLAS_STATUS_SUCCESS;
     }
-    const int rc = ChooseRc(n, n);
-    if (rc <= 0) {
+    if (ChooseRc(n, n) <= 0) {
         LogError("aclblasSgetriBatched", "n=%d is too large", n);
         return ACLBLAS_STATUS_NOT_SUPPORTED;
     }
     if (!HandleHasDevice(handle)) {
         return ACLBLAS_STATUS_EXECUTION_FAILED;
     }
-    LaunchSolve<kModeInv>(handle->stream, RoundRc(rc), Aarray, n, lda, PivotArray, Carray, ldc, n, infoArray,
-                          batchSize);
-    return LaunchStatus("aclblasSgetriBatched");
+    return GetriBatchedLaunch(handle->stream, n, Aarray, lda, PivotArray, Carray, ldc, infoArray, batchSize);
 }

@@ -18,4 +18,9 @@ void LaunchGetrfBlock(hipStream_t st, float* const* A, int n, int lda, int* ipiv
 bool LaunchGetrsSmall(hipStream_t st, bool trans, int n, int nrhs, const float* const* A, int lda, const int* piv,
                       float* const* B, int ldb, int batch);
 
+// Inverse C = A^-1 from LU factors, n <= 64 (getrs_small.hip): info[b] = j+1
+// for the zero u_jj with the largest j; false when the shape is not covered.
+bool LaunchGetriSmall(hipStream_t st, int n, const float* const* A, int lda, const int* piv, float* const* C,
+                      int ldc, int* info, int batch);
+
 }  // namespace aclblas

@@ -169,6 +169,18 @@ def getri_batched(LU, ipiv, ldc=None, nthreads=None):
     return C, info
 
 
+def matinv_batched(A, lda_inv=None, nthreads=None):
+    """Ainv = A^-1 as the reference composes it (blas/matinv_batched/arch35/
+    matinv_batched_host.cpp:165-195): pivoted getrf on a copy of A, then getri
+    from those factors; the returned info is getri's, which replaces getrf's.
+    Pivots after a zero pivot stay 0 (no interchange).  A (batch, n, lda);
+    returns (Ainv (batch, n, lda_inv), info (batch,))."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    n = A.shape[1]
+    LU, ipiv, _ = getrf_batched(np.ascontiguousarray(A[:, :, :n]).copy(), pivot=True, nthreads=nthreads)
+    return getri_batched(LU, ipiv, ldc=lda_inv or n, nthreads=nthreads)
+
+
 def time_getrf_getrs(A, B, nthreads, reps=3):
     """Median seconds of one getrf+getrs pass over A (batch, n, n), B (batch, nrhs, n)."""
     A = np.ascontiguousarray(A, dtype=np.float32)

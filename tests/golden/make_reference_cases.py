@@ -16,6 +16,7 @@ TABLES = {
     "sgetrf_batched.csv": "test/getrf_batched/sgetrf_batched/arch35/sgetrf_batched_test.csv",
     "getrs_batched.csv": "test/getrs_batched/arch35/getrs_batched_test.csv",
     "sgetri_batched.csv": "test/getri_batched/sgetri_batched/arch35/sgetri_batched_test.csv",
+    "smatinv_batched.csv": "test/matinv_batched/arch35/matinv_batched_test.csv",
     "sgemm_ex.csv": "test/sgemm_ex/arch35/sgemm_ex_test.csv",
     "gemm_ex.csv": "test/gemm/gemm_ex/arch35/gemm_ex_test.csv",
     "gemm_batched_ex.csv": "test/gemm_batched_ex/arch35/gemm_batched_ex_test.csv",

@@ -187,6 +187,28 @@ def test_getri_null_handle():
     assert B.sgetri_batched(None, 4, DUMMY, 4, None, DUMMY, 4, DUMMY, 1) == status_code("HANDLE_IS_NULLPTR")
 
 
+MATINV_ROWS = _lapack_rows("smatinv_batched.csv", "n", "batch_size")
+
+
+@pytest.mark.parametrize("row", MATINV_ROWS, ids=[r["case_name"] for r in MATINV_ROWS])
+def test_matinv_argument_rows(cpu_handle, row):
+    kind = row["matrix_type"]
+    n, batch = int(row["n"]), int(row["batch_size"])
+    if status_code(row["expect_result"]) == 0:
+        # no-op rows: the reference passes null arrays (matinv_batched_test.cpp:96-103)
+        A = Ainv = info = None
+    else:
+        A = None if kind in ("NULLPTR_A", "NULLPTR_ALL") else DUMMY
+        Ainv = None if kind in ("NULLPTR_AINV", "NULLPTR_ALL") else DUMMY
+        info = None if kind in ("NULLPTR_INFO", "NULLPTR_ALL") else DUMMY
+    st = B.smatinv_batched(cpu_handle, n, A, int(row["lda"]), Ainv, int(row["lda_inv"]), info, batch)
+    assert st == status_code(row["expect_result"])
+
+
+def test_matinv_null_handle():
+    assert B.smatinv_batched(None, 4, DUMMY, 4, DUMMY, 4, DUMMY, 1) == status_code("HANDLE_IS_NULLPTR")
+
+
 # ------------------------------------------------------------------ GEMM rows
 def _gemm_degenerate(r):
     m, n = int(r["m"]), int(r["n"])

@@ -234,3 +234,32 @@ def test_getri_reference_cases(handle, row):
             assert linfo == 0
             ok, detail = O.mixed_tolerance(Ch[b, :, :n].T, ref)
             assert ok, detail
+
+
+@pytest.mark.parametrize("n", [1, 7, 16, 17, 31, 32, 33, 63, 64, 65, 100])
+@pytest.mark.parametrize("pivot", [True, False])
+def test_getri_sizes_and_zero_diagonals(handle, n, pivot):
+    # n <= 64 runs the register kernel, larger n the LDS kernel: both bit-exact
+    # against the oracle; members 1 and 2 get zero diagonal entries (info is the
+    # largest such j + 1 and the U solve is skipped)
+    import torch
+    batch, ldc = 6, n + 3
+    A = O.normal((batch, n, n), 40 + n, n=n, batch=batch, shift=4.0)
+    LU, ipiv, _ = O.getrf_batched(A.copy(), pivot=pivot)
+    LU[1, 0, 0] = 0.0
+    LU[2, n // 2, n // 2] = 0.0
+    LU[2, n - 1, n - 1] = 0.0
+    dLU = to_dev(LU)
+    C = torch.full((batch, n, ldc), 9.0, dtype=torch.float32, device="cuda")
+    info = torch.full((batch,), -1, dtype=torch.int32, device="cuda")
+    dpiv = to_dev(ipiv) if pivot else None
+    pa, pc = B.ptr_array(dLU), B.ptr_array(C)
+    st = B.sgetri_batched(handle.value, n, pa.data_ptr(), n, dpiv.data_ptr() if pivot else None, pc.data_ptr(),
+                          ldc, info.data_ptr(), batch)
+    assert st == 0
+    Ch, ih = to_host(C), to_host(info)
+    gC, ginfo = O.getri_batched(LU, ipiv, ldc=ldc)
+    assert ginfo[1] == 1 and ginfo[2] == n and ginfo[0] == 0
+    assert np.array_equal(ih, ginfo)
+    assert np.array_equal(Ch[:, :, :n], gC[:, :, :n], equal_nan=True)
+    assert np.all(Ch[:, :, n:] == 9.0)

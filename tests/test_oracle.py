@@ -203,6 +203,59 @@ def test_getri_info_is_largest_zero_diagonal():
     assert np.array_equal(C[0], np.eye(n, dtype=np.float32))
 
 
+# ------------------------------------------------------------------ matinv
+def test_matinv_readme_example_closed_form():
+    # reference blas/matinv_batched/README.md example: a_ii = 2, a_ij = 0.1 (n 16,
+    # batch 4), i.e. A = 1.9 I + 0.1 J; Sherman-Morrison gives
+    # A^-1 = I / 1.9 - 0.1 / (1.9 (1.9 + 1.6)) J
+    n, batch = 16, 4
+    A = np.full((batch, n, n), 0.1, np.float32)
+    for b in range(batch):
+        np.fill_diagonal(A[b], 2.0)
+    C, info = O.matinv_batched(A)
+    assert np.all(info == 0)
+    exact = np.eye(n) / 1.9 - 0.1 / (1.9 * 3.5) * np.ones((n, n))
+    bound = 4 * n * 2.0 ** -24 * np.abs(exact).max()   # cond(A) = 3.5 / 1.9 < 2
+    for b in range(batch):
+        assert np.abs(C[b].T - exact).max() < bound
+
+
+def test_matinv_matches_fp64_inverse_and_keeps_padding_layout():
+    rng = np.random.default_rng(5)
+    for n, lda, ldi in ((1, 1, 1), (5, 9, 7), (32, 32, 40)):
+        A = np.zeros((3, n, lda), np.float32)
+        A[:, :, :n] = rng.standard_normal((3, n, n)) + 4 * np.eye(n)
+        C, info = O.matinv_batched(A, lda_inv=ldi)
+        assert C.shape == (3, n, ldi) and np.all(info == 0)
+        for b in range(3):
+            ref = np.linalg.inv(A[b, :, :n].T.astype(np.float64))
+            ok, detail = O.mixed_tolerance(C[b, :, :n].T, ref)
+            assert ok, detail
+            # the composition is literally getrf then getri
+            LU, piv, _ = O.getrf_batched(A[b:b + 1, :, :n].copy())
+            C2, _ = O.getri_batched(LU, piv, ldc=ldi)
+            assert np.array_equal(C[b], C2[0])
+
+
+def test_matinv_info_is_the_inverse_rule():
+    # column 1 is zero: the LU stops at k = 1 (its info would be 2, LAPACK's
+    # too); the inverse then finds u_33 = 0 in the unfactored part and reports
+    # the largest zero diagonal, 4 -- the reference's composition returns the
+    # inverse's info (matinv_batched_host.cpp:157-186)
+    A = np.zeros((1, 4, 4), np.float32)
+    A[0, 0, 0] = 1.0
+    A[0, 2, 2] = 1.0
+    C, info = O.matinv_batched(A)
+    assert info[0] == 4
+    _, _, finfo = O.getrf_batched(A.copy())
+    assert finfo[0] == 2
+    # zero column only: both rules give its index
+    A2 = (np.eye(6, dtype=np.float32) * 3 + 0.5)[None].copy()
+    A2[0, 2, :] = 0.0
+    _, info2 = O.matinv_batched(A2)
+    assert info2[0] == 3 == lapack.sgetrf(A2[0].T)[2]
+
+
 # ------------------------------------------------------------------ GEMM
 def test_gemm_known_answer():
     # reference blas/gemm_ex/README.md:184-188
