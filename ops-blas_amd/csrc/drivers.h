@@ -15,6 +15,12 @@ namespace aclblas {
 aclblasStatus_t GetrfBatchedLaunch(const _aclblas_handle* h, int n, float* const* A, int lda, int* piv, int* info,
                                    int batch);
 
+// Pivoted LU of n <= 32 matrices, out of place: A read, the factors written
+// to out + b n^2 (leading dimension n), &out[b n^2] to table[b], all n pivots
+// of each matrix written (0 after a zero pivot), getrf.hip.
+aclblasStatus_t GetrfOutOfPlaceLaunch(const _aclblas_handle* h, int n, const float* const* A, int lda, float* out,
+                                      float** table, int* piv, int* info, int batch);
+
 // op(A) X = B with LU factors, B overwritten (trans: A^T), getrs.hip.
 aclblasStatus_t GetrsBatchedLaunch(hipStream_t st, bool trans, int n, int nrhs, const float* const* A, int lda,
                                    const int* piv, float* const* B, int ldb, int batch);

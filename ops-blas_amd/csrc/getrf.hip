@@ -116,19 +116,24 @@ __global__ __launch_bounds__(kGlobalThreads) void GetrfGlobalKernel(
     }
 }
 
+// Rows per lane of the n <= 64 LU: the tuned choice for (S, batch), else two
+// rows per lane while that still leaves two waves per SIMD (they halve the
+// per-matrix instruction count but also the waves).
+int WaveRows(const _aclblas_handle* h, int n, int batch)
+{
+    const int cus = h->cu_count > 0 ? h->cu_count : 256;
+    const int S = n <= 8 ? 8 : (n <= 16 ? 16 : (n <= 32 ? 32 : 64));
+    const long long wavesTwoRows = static_cast<long long>(batch) * S / 128;
+    const int builtIn = (S > 8 && wavesTwoRows >= 8ll * cus) ? 2 : 1;
+    return tune::Get(h, tune::kOpLu, tune::LuBucket(S, batch, cus), tune::Choice{builtIn, 1}).a;
+}
+
 template <bool PIVOT>
 void LaunchGetrf(const _aclblas_handle* h, float* const* A, int n, int lda, int* ipiv, int* info, int batch)
 {
     const hipStream_t st = h->stream;
     if (n <= 64) {
-        // two rows per lane halve the per-matrix instruction count but also the
-        // waves: built in only while that still leaves two waves per SIMD
-        const int cus = h->cu_count > 0 ? h->cu_count : 256;
-        const int S = n <= 8 ? 8 : (n <= 16 ? 16 : (n <= 32 ? 32 : 64));
-        const long long wavesTwoRows = static_cast<long long>(batch) * S / 128;
-        const int builtIn = (S > 8 && wavesTwoRows >= 8ll * cus) ? 2 : 1;
-        const tune::Choice c = tune::Get(h, tune::kOpLu, tune::LuBucket(S, batch, cus), tune::Choice{builtIn, 1});
-        LaunchGetrfWave<PIVOT>(st, A, n, lda, ipiv, info, batch, c.a);
+        LaunchGetrfWave<PIVOT>(st, A, n, lda, ipiv, info, batch, WaveRows(h, n, batch));
     } else if (n <= 256) {
         LaunchGetrfBlock<PIVOT>(st, A, n, lda, ipiv, info, batch);
     } else {
@@ -160,6 +165,16 @@ aclblasStatus_t ValidateGetrf(int n, float* const* A, int lda, const int* piv, c
 }
 
 }  // namespace
+
+aclblasStatus_t GetrfOutOfPlaceLaunch(const _aclblas_handle* h, int n, const float* const* A, int lda, float* out,
+                                      float** table, int* piv, int* info, int batch)
+{
+    if (n <= 0 || n > 32) {
+        return ACLBLAS_STATUS_NOT_SUPPORTED;
+    }
+    LaunchGetrfWaveOutOfPlace(h->stream, A, n, lda, out, table, piv, info, batch, WaveRows(h, n, batch));
+    return LaunchStatus("aclblasSgetrfBatched");
+}
 
 // Internal entry shared with the solve/refinement drivers.
 aclblasStatus_t GetrfBatchedLaunch(const _aclblas_handle* h, int n, float* const* A, int lda, int* piv, int* info,

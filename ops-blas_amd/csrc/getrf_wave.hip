@@ -17,6 +17,10 @@
 // its final position, at the end.  R = 2 halves the segment width, so a wave
 // carries twice the matrices and the reduction, broadcast and bookkeeping
 // instructions per matrix halve, for twice the update work per lane.
+// OOP (the batched inverse's LU): the factors go to a packed n x n block per
+// matrix (out + b n^2, leading dimension n) instead of back into A, the block's
+// address into table[b], and every pivot entry is written (0 after a zero
+// pivot), so the input is read once and needs no separate copy.
 #include "device_common.h"
 #include "internal.h"
 #include "lapack_kernels.h"
@@ -28,9 +32,10 @@ using dev::kWave;
 
 constexpr int kWavesPerBlock = 4;
 
-template <int S, int R, bool PIVOT>
+template <int S, int R, bool PIVOT, bool OOP>
 __global__ __launch_bounds__(kWave* kWavesPerBlock) void GetrfWaveKernel(
-    float* const* __restrict__ Aarray, int n, int lda, int* __restrict__ ipiv, int* __restrict__ info, int batch)
+    float* const* __restrict__ Aarray, int n, int lda, int* __restrict__ ipiv, int* __restrict__ info, int batch,
+    float* __restrict__ out, float** __restrict__ table)
 {
     constexpr int L = S / R;            // lanes per matrix
     constexpr int kMpw = kWave / L;     // matrices per wave
@@ -195,13 +200,22 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void GetrfWaveKernel(
         });
     });
 
+    GlobalFloat* O = A;
+    size_t ldo = static_cast<size_t>(lda);
+    if (OOP && live) {
+        O = (GlobalFloat*)(out + static_cast<size_t>(b) * n * n);
+        ldo = static_cast<size_t>(n);
+        if (r == 0) {
+            table[b] = out + static_cast<size_t>(b) * n * n;
+        }
+    }
     dev::StaticFor<0, R>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
         if (real[i]) {
             dev::StaticFor<0, S>([&](auto jc) {
                 constexpr int j = decltype(jc)::value;
                 if (j < n) {
-                    A[pos[i] + static_cast<size_t>(j) * lda] = a[i][j];
+                    O[pos[i] + static_cast<size_t>(j) * ldo] = a[i][j];
                 }
             });
         }
@@ -215,6 +229,8 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void GetrfWaveKernel(
             constexpr int i = decltype(ic)::value;
             if (r + i * L < written) {
                 ipiv[static_cast<size_t>(b) * n + r + i * L] = myPiv[i];
+            } else if (OOP && r + i * L < n) {
+                ipiv[static_cast<size_t>(b) * n + r + i * L] = 0;
             }
         });
     }
@@ -223,13 +239,15 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void GetrfWaveKernel(
     }
 }
 
-template <int S, int R, bool PIVOT>
-void Launch(hipStream_t st, float* const* A, int n, int lda, int* ipiv, int* info, int batch)
+template <int S, int R, bool PIVOT, bool OOP = false>
+void Launch(hipStream_t st, float* const* A, int n, int lda, int* ipiv, int* info, int batch,
+            float* out = nullptr, float** table = nullptr)
 {
     constexpr int kPerBlock = kWavesPerBlock * (kWave * R / S);
     const dim3 grid((batch + kPerBlock - 1) / kPerBlock);
     const dim3 block(kWave * kWavesPerBlock);
-    hipLaunchKernelGGL((GetrfWaveKernel<S, R, PIVOT>), grid, block, 0, st, A, n, lda, ipiv, info, batch);
+    hipLaunchKernelGGL((GetrfWaveKernel<S, R, PIVOT, OOP>), grid, block, 0, st, A, n, lda, ipiv, info, batch, out,
+                       table);
 }
 
 }  // namespace
@@ -257,6 +275,25 @@ void LaunchGetrfWave(hipStream_t st, float* const* A, int n, int lda, int* ipiv,
         } else {
             Launch<64, 1, PIVOT>(st, A, n, lda, ipiv, info, batch);
         }
+    }
+}
+
+void LaunchGetrfWaveOutOfPlace(hipStream_t st, const float* const* A, int n, int lda, float* out, float** table,
+                               int* ipiv, int* info, int batch, int rows)
+{
+    float* const* in = const_cast<float* const*>(A);  // read only in this mode
+    if (n <= 8) {
+        Launch<8, 1, true, true>(st, in, n, lda, ipiv, info, batch, out, table);
+    } else if (n <= 16) {
+        if (rows == 2) {
+            Launch<16, 2, true, true>(st, in, n, lda, ipiv, info, batch, out, table);
+        } else {
+            Launch<16, 1, true, true>(st, in, n, lda, ipiv, info, batch, out, table);
+        }
+    } else if (rows == 2) {
+        Launch<32, 2, true, true>(st, in, n, lda, ipiv, info, batch, out, table);
+    } else {
+        Launch<32, 1, true, true>(st, in, n, lda, ipiv, info, batch, out, table);
     }
 }
 

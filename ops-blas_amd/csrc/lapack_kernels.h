@@ -10,6 +10,12 @@ namespace aclblas {
 template <bool PIVOT>
 void LaunchGetrfWave(hipStream_t st, float* const* A, int n, int lda, int* ipiv, int* info, int batch, int rows);
 
+// Pivoted LU, n <= 32, out of place (getrf_wave.hip): reads A, writes the
+// factors to out + b n^2 (leading dimension n), &out[b n^2] to table[b] and
+// all n pivots of each matrix (0 after a zero pivot).
+void LaunchGetrfWaveOutOfPlace(hipStream_t st, const float* const* A, int n, int lda, float* out, float** table,
+                               int* ipiv, int* info, int batch, int rows);
+
 // LU, 64 < n <= 256 (getrf_block.hip)
 template <bool PIVOT>
 void LaunchGetrfBlock(hipStream_t st, float* const* A, int n, int lda, int* ipiv, int* info, int batch);

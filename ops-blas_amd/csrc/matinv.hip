@@ -7,55 +7,24 @@
 // active workspace without growing it, EXECUTION_FAILED when it is smaller,
 // pivoted LU, then the out-of-place inverse whose info replaces the LU's).
 //
-// MI355X design: three launches on the handle's stream instead of five.  One
-// kernel copies each A[i] into a packed n x n block of the workspace and writes
-// that block's address into the workspace's pointer table (one wave per matrix,
-// coalesced on both sides); the register-resident LU factors the blocks in
-// place; the inverse kernel reads the factors and writes Ainv[i] directly.  The
-// reference copies A into Ainv, factors Ainv and copies the factors out again
-// before inverting; the factors, pivots, inverse and info are the same, with
-// one HBM pass fewer.  The pivots start at zero, so the interchanges that a
+// MI355X design: two launches on the handle's stream instead of five.  The
+// register-resident LU reads each A[i] and writes its factors to a packed
+// n x n block of the workspace, the block's address to the workspace's
+// pointer table and all n pivots; the inverse kernel (one lane per column)
+// reads the factors and writes Ainv[i] directly.  The reference copies A into
+// Ainv, factors Ainv and copies the factors out again before inverting; the
+// factors, pivots, inverse and info are the same, with three HBM passes fewer.
+// The pivots after a zero pivot are written as 0, so the interchanges that a
 // singular member's LU never reaches are skipped by the inverse (the reference
 // leaves them as whatever the workspace held).  Nothing is allocated, so the
 // call can be captured.
-#include "device_common.h"
 #include "drivers.h"
 #include "internal.h"
 
 namespace aclblas {
 namespace {
 
-using dev::kWave;
-
 constexpr int kMaxMatinvN = 32;
-constexpr int kWavesPerBlock = 4;
-
-__global__ __launch_bounds__(kWave* kWavesPerBlock) void StageBlocksKernel(
-    const float* const* __restrict__ A, int n, int lda, float* __restrict__ blocks, float** __restrict__ table,
-    int* __restrict__ piv, int batch)
-{
-    const int lane = dev::LaneId();
-    const int b = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x) / kWave;
-    if (b >= batch) {
-        return;
-    }
-    const int nn = n * n;
-    float* dst = blocks + static_cast<size_t>(b) * nn;
-    const float* src = A[b];
-    for (int e = lane; e < nn; e += kWave) {
-        const int j = e / n;
-        const int i = e - j * n;
-        dst[e] = src[i + static_cast<size_t>(j) * lda];
-    }
-    if (lane == 0) {
-        table[b] = dst;
-    }
-    // the LU leaves the pivots after a zero pivot unwritten; zero (= no
-    // interchange for the inverse) keeps a singular member's output defined
-    if (lane < n) {
-        piv[static_cast<size_t>(b) * n + lane] = 0;
-    }
-}
 
 // reference matinv_batched_host.cpp:42-81 (same checks, same order)
 aclblasStatus_t ValidateMatinv(int n, int lda, int ldaInv, int batch, const float* const* A, float* const* Ainv,
@@ -117,14 +86,7 @@ extern "C" aclblasStatus_t aclblasSmatinvBatched(
     float* blocks = reinterpret_cast<float*>(ws + tableBytes);
     int* piv = reinterpret_cast<int*>(ws + tableBytes + blockBytes);
 
-    const unsigned grid = static_cast<unsigned>((batch + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL(StageBlocksKernel, dim3(grid), dim3(kWave * kWavesPerBlock), 0, handle->stream, A, n, lda,
-                       blocks, table, piv, batchSize);
-    st = LaunchStatus("aclblasSmatinvBatched");
-    if (st != ACLBLAS_STATUS_SUCCESS) {
-        return st;
-    }
-    st = GetrfBatchedLaunch(handle, n, table, n, piv, info, batchSize);
+    st = GetrfOutOfPlaceLaunch(handle, n, A, lda, blocks, table, piv, info, batchSize);
     if (st != ACLBLAS_STATUS_SUCCESS) {
         return st;
     }
