@@ -1,5 +1,5 @@
-"""HIP graph capture of the calls that stage a per-problem table (mixed-order solve, grouped GEMM) and of a
-plain LU + solve sequence.
+"""HIP graph capture of the calls that stage a per-problem table (mixed-order solve, grouped GEMM), of a
+plain LU + solve sequence and of the batched inverse.
 
 Each call is captured once on a dedicated stream (the handle bound to it before capture, so capture sees
 no stream synchronisation and no workspace growth), then the graph is replayed several times with new
@@ -165,6 +165,48 @@ def test_lu_solve_sequence_graph_replays():
                 assert st == 0
             stream.synchronize()
             assert torch.equal(LU, LU2) and torch.equal(piv, piv2) and torch.equal(X, X2), rep
+    finally:
+        torch.cuda.synchronize()
+        h.destroy()
+
+
+def test_matinv_graph_replays():
+    # the batched inverse allocates nothing and records no memset: two kernels
+    # (out-of-place LU into the handle's workspace, inverse into Ainv)
+    dev = torch.device("cuda")
+    stream = torch.cuda.Stream()
+    h = B.Handle()
+    assert h.set_stream(stream.cuda_stream) == 0
+    try:
+        batch, n = 512, 32
+        rng = np.random.default_rng(9)
+        dA = torch.from_numpy((rng.standard_normal((batch, n, n)) + 4 * np.eye(n)).astype(np.float32)).to(dev)
+        C = torch.zeros_like(dA)
+        C2 = torch.zeros_like(dA)
+        info = torch.full((batch,), -1, dtype=torch.int32, device=dev)
+        info2 = torch.full_like(info, -1)
+        pa, pc, pc2 = B.ptr_array(dA), B.ptr_array(C), B.ptr_array(C2)
+
+        def inv(out, inf):
+            return B.smatinv_batched(h.value, n, pa.data_ptr(), n, out.data_ptr(), n, inf.data_ptr(), batch)
+
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream, capture_error_mode="relaxed"):
+            assert inv(pc, info) == 0
+        for rep in range(4):
+            newA = rng.standard_normal((batch, n, n)) + 4 * np.eye(n)
+            if rep == 2:
+                newA[7, 3, :] = 0.0                 # a singular member: info 4
+            with torch.cuda.stream(stream):
+                dA.copy_(torch.from_numpy(newA.astype(np.float32)).to(dev))
+                C.zero_()
+                g.replay()
+                assert inv(pc2, info2) == 0        # eager reference on the same inputs
+            stream.synchronize()
+            assert torch.equal(C, C2), "replay %d differs from the eager call" % rep
+            assert torch.equal(info, info2)
+            assert int(info[7]) == (4 if rep == 2 else 0)
     finally:
         torch.cuda.synchronize()
         h.destroy()
