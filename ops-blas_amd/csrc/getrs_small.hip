@@ -11,7 +11,9 @@
 // fmaf per lane and right-hand side: no
 // LDS, no barriers.  Row interchanges: every lane traces its final position
 // back through the swap list to find which row of B it loads (trans N) or
-// stores (trans T).  Element by element the operations and their order are
+// stores (trans T); for trans N each lane loads its own row and the rows are
+// then permuted inside the segment with ds_bpermute, so that the load of B does
+// not wait for the pivots.  Element by element the operations and their order are
 // those of the general kernel, so both give identical results.
 #include "device_common.h"
 #include "internal.h"
@@ -72,6 +74,15 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void SolveSmallKernel(
         });
     }
 
+    // this lane's own row of B (trans N: permuted below, inside the segment, so
+    // the load does not wait for the pivots)
+    float* B = live ? Barray[b] : nullptr;
+    float x[RC];
+    dev::StaticFor<0, RC>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        x[c] = (real && c < rc) ? B[r + static_cast<size_t>(c0 + c) * ldb] : 0.0f;
+    });
+
     // row of B that lands at position r after the interchanges
     int src = r;
     if (ipiv != nullptr && real) {
@@ -90,13 +101,14 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void SolveSmallKernel(
         });
     }
 
-    float* B = live ? Barray[b] : nullptr;
-    float x[RC];
-    dev::StaticFor<0, RC>([&](auto cc) {
-        constexpr int c = decltype(cc)::value;
-        const int row = TRANS ? r : src;
-        x[c] = (real && c < rc) ? B[row + static_cast<size_t>(c0 + c) * ldb] : 0.0f;
-    });
+    if (!TRANS) {
+        // x <- (P b): the value of lane src of this segment (every lane active here)
+        const int from = (lane - r + src) * 4;
+        dev::StaticFor<0, RC>([&](auto cc) {
+            constexpr int c = decltype(cc)::value;
+            x[c] = __int_as_float(__builtin_amdgcn_ds_bpermute(from, __float_as_int(x[c])));
+        });
+    }
 
     if (!TRANS) {
         // L (unit lower): x_i -= l_ij x_j for i > j, j ascending
