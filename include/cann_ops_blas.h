@@ -155,6 +155,22 @@ aclblasStatus_t aclblasSgemmGroupedBatched(
     const int* ldbArray, const float* betaArray, float* const* Carray, const int* ldcArray,
     const int* groupSizeArray);
 
+/* reference include/cann_ops_blas.h:461-467.  Grouped GEMM over FP16 x FP16 ->
+ * FP16 (COMPUTE_16F or _32F), BF16 x BF16 -> BF16 (_32F) and FP8 E4M3/E5M2 in
+ * any mix -> FP16 (_32F); other combinations NOT_SUPPORTED before any other
+ * check.  trans/m/n/k/lda/ldb/ldc/groupSize and alphaArray/betaArray are HOST
+ * arrays of groupCount entries (alpha/beta FP16 under COMPUTE_16F, float
+ * otherwise); Aarray/Barray/Carray are DEVICE arrays of device pointers, the
+ * problems of group 0 first.  Products accumulate in FP32; FP16 results round
+ * to nearest with ties away from zero, BF16 results to nearest even. */
+aclblasStatus_t aclblasGemmGroupedBatchedEx(
+    aclblasHandle_t handle, const aclblasOperation_t transaArray[], const aclblasOperation_t transbArray[],
+    const int mArray[], const int nArray[], const int kArray[], const void* alphaArray,
+    const void* const Aarray[], aclDataType Atype, const int ldaArray[],
+    const void* const Barray[], aclDataType Btype, const int ldbArray[],
+    const void* betaArray, void* const Carray[], aclDataType Ctype, const int ldcArray[],
+    int groupCount, const int groupSize[], aclblasComputeType_t computeType);
+
 /* ------------------------------------------------------------------------ */
 /* Batched LAPACK (FP32)                                                     */
 /* ------------------------------------------------------------------------ */

@@ -61,6 +61,7 @@ _SIGNATURES = {
     "aclblasSgetrsBatched": [_P, _I, _I, _I, _P, _I, _P, _P, _I, _P, _I],
     "aclblasSgetriBatched": [_P, _I, _P, _I, _P, _P, _I, _P, _I],
     "aclblasSmatinvBatched": [_P, _I, _P, _I, _P, _I, _P, _I],
+    "aclblasGemmGroupedBatchedEx": [_P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _I, _P, _P, _P, _I, _P, _I, _P, _I],
     "aclblasSgetrsBatchedAdjoint": [_P, _I, _I, _P, _I, _P, _P, _I, _P, _I, _P, _I, _I],
     "aclblasSgemmGroupedBatched": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "aclblasSgesvBatchedMixed": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, _I, _P, _P, _P],
@@ -210,6 +211,24 @@ def sgemm_grouped_batched(h, transa, transb, m, n, k, alpha, A, lda, Bp, ldb, be
 def sgemm_grouped_batched_raw(h, group_count, *arrays):
     ptrs = [ctypes.cast(a, ctypes.c_void_p) if a is not None else None for a in arrays]
     return load().aclblasSgemmGroupedBatched(h, group_count, *ptrs)
+
+
+def gemm_grouped_batched_ex(h, transa, transb, m, n, k, alpha, Aarray, atype, lda, Barray, btype, ldb, beta,
+                            Carray, ctype, ldc, group_size, compute, group_count=None):
+    """transa .. ldc, group_size: Python lists (host arrays) or None; alpha / beta: lists of floats (COMPUTE_32F)
+    or ctypes arrays of FP16 bit patterns (COMPUTE_16F); Aarray / Barray / Carray: device addresses of device
+    pointer arrays holding every problem of every group, group after group."""
+    if group_count is None:
+        group_count = len(group_size) if group_size is not None else 0
+    ints = lambda v: _host_array(ctypes.c_int, v)
+    arrs = [ints(transa), ints(transb), ints(m), ints(n), ints(k), _host_array(ctypes.c_float, alpha)]
+    ptrs = [ctypes.cast(a, ctypes.c_void_p) if a is not None else None for a in arrs]
+    lda_a, ldb_a, beta_a, ldc_a, gs_a = ints(lda), ints(ldb), _host_array(ctypes.c_float, beta), ints(ldc), \
+        ints(group_size)
+    cast = lambda a: ctypes.cast(a, ctypes.c_void_p) if a is not None else None
+    return load().aclblasGemmGroupedBatchedEx(h, *ptrs, Aarray, atype, cast(lda_a), Barray, btype, cast(ldb_a),
+                                              cast(beta_a), Carray, ctype, cast(ldc_a), int(group_count),
+                                              cast(gs_a), compute)
 
 
 def sgesv_batched_mixed(h, n, nrhs, A, lda, Bp, ldb, X, ldx, tol, max_iter, iters=None, berr=None, info=None):

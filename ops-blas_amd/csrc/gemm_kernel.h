@@ -483,8 +483,27 @@ __device__ __forceinline__ uint16_t ToF16(float f)
     return __builtin_bit_cast(uint16_t, h);
 }
 __device__ __forceinline__ float FromF16(uint16_t h) { return static_cast<float>(__builtin_bit_cast(_Float16, h)); }
+// FP16 rounding to nearest with ties away from zero: the nearest-even result,
+// moved one step away from zero when f lies exactly halfway and nearest-even
+// went toward zero (both differences below are exact in FP32).
+__device__ __forceinline__ uint16_t ToF16Away(float f)
+{
+    const uint16_t h = ToF16(f);
+    const float back = FromF16(h);
+    const float mag = __builtin_fabsf(f);
+    const float hm = __builtin_fabsf(back);
+    if (mag > hm && (h & 0x7FFFu) < 0x7C00u) {
+        const uint16_t up = static_cast<uint16_t>(h + 1u);  // same sign, next magnitude
+        if (mag - hm == __builtin_fabsf(FromF16(up)) - mag) {
+            return up;
+        }
+    }
+    return h;
+}
 
-enum OutKind { kOutF32 = 0, kOutBF16 = 1, kOutF16 = 2 };
+// kOutF16A: FP16 with ties rounded away from zero (the grouped 16-bit/FP8 GEMM's
+// epilogue rounding), no FP8 non-finite rule.
+enum OutKind { kOutF32 = 0, kOutBF16 = 1, kOutF16 = 2, kOutF16A = 3 };
 
 template <int OK>
 __device__ __forceinline__ float LoadC(const void* C, size_t idx)
@@ -501,6 +520,8 @@ __device__ __forceinline__ void StoreC(void* C, size_t idx, float v)
         static_cast<float*>(C)[idx] = v;
     } else if (OK == kOutBF16) {
         static_cast<uint16_t*>(C)[idx] = ToBf16(v);
+    } else if (OK == kOutF16A) {
+        static_cast<uint16_t*>(C)[idx] = ToF16Away(v);
     } else {
         static_cast<uint16_t*>(C)[idx] = ToF16(v);
     }
@@ -537,7 +558,9 @@ __device__ __forceinline__ void StoreC4(void* C, size_t idx, bool full, int vali
         } else {
             uint16_t h[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) h[e] = OK == kOutBF16 ? ToBf16(v[e]) : ToF16(v[e]);
+            for (int e = 0; e < 4; ++e) {
+                h[e] = OK == kOutBF16 ? ToBf16(v[e]) : (OK == kOutF16A ? ToF16Away(v[e]) : ToF16(v[e]));
+            }
             *reinterpret_cast<uint2*>(static_cast<uint16_t*>(C) + idx) =
                 make_uint2(h[0] | (static_cast<uint32_t>(h[1]) << 16), h[2] | (static_cast<uint32_t>(h[3]) << 16));
         }
@@ -749,7 +772,7 @@ __device__ __forceinline__ void GemmTile(const TileJob& q, int m0, int n0, uint8
         __syncthreads();
     }
 
-    StoreTileC<OK, TM, TN, kFp8>(q, acc, m0 + wm, n0 + wn, lane);
+    StoreTileC<OK, TM, TN, kFp8 && OK != kOutF16A>(q, acc, m0 + wm, n0 + wn, lane);
 }
 
 // One problem (or a uniform batch through pointer arrays): one workgroup per

@@ -21,6 +21,7 @@ TABLES = {
     "gemm_ex.csv": "test/gemm/gemm_ex/arch35/gemm_ex_test.csv",
     "gemm_batched_ex.csv": "test/gemm_batched_ex/arch35/gemm_batched_ex_test.csv",
     "gemm_grouped_batched.csv": "test/gemm_grouped_batched/arch35/gemm_grouped_batched_test.csv",
+    "gemm_grouped_batched_ex.csv": "test/gemm_grouped_batched_ex/arch35/gemm_grouped_batched_ex_test.csv",
 }
 
 

@@ -209,6 +209,48 @@ def test_matinv_null_handle():
     assert B.smatinv_batched(None, 4, DUMMY, 4, DUMMY, 4, DUMMY, 1) == status_code("HANDLE_IS_NULLPTR")
 
 
+GROUPED_EX_ROWS = [r for r in load_cases("gemm_grouped_batched_ex.csv")
+                   if status_code(r["expect_result"]) != 0 or int(r["group_count"]) == 0]
+
+
+@pytest.mark.parametrize("row", GROUPED_EX_ROWS, ids=[r["case_name"] for r in GROUPED_EX_ROWS])
+def test_grouped_ex_status_rows(cpu_handle, row):
+    gc = int(row["group_count"])
+    one = [1] * max(gc, 1)
+    st = B.gemm_grouped_batched_ex(cpu_handle, [B.OP_N] * len(one), [B.OP_N] * len(one), [16] * len(one),
+                                   [16] * len(one), [32] * len(one), [1.0] * len(one), DUMMY,
+                                   B.DTYPES[row["Atype"]], [16] * len(one), DUMMY, B.DTYPES[row["Btype"]],
+                                   [32] * len(one), [0.0] * len(one), DUMMY, B.DTYPES[row["Ctype"]],
+                                   [16] * len(one), one, B.COMPUTE[row["computeType"]], group_count=gc)
+    assert st == status_code(row["expect_result"])
+
+
+def test_grouped_ex_argument_checks(cpu_handle):
+    inv, ns = status_code("INVALID_VALUE"), status_code("NOT_SUPPORTED")
+    F16, C32 = B.ACL_FLOAT16, B.COMPUTE["COMPUTE_32F"]
+
+    def call(h=cpu_handle, ta=(B.OP_N,), m=(16,), k=(16,), lda=(16,), ldb=(16,), ldc=(16,), size=(1,), gc=1,
+             A=DUMMY, at=F16, ct=F16, comp=C32, alpha=(1.0,)):
+        return B.gemm_grouped_batched_ex(h, list(ta), [B.OP_N], list(m), [16], list(k), list(alpha), A, at,
+                                         list(lda), DUMMY, at, list(ldb), [0.0], DUMMY, ct, list(ldc), list(size),
+                                         comp, group_count=gc)
+
+    assert call(h=None) == status_code("HANDLE_IS_NULLPTR")
+    assert call(ct=B.ACL_BF16) == ns                       # FP16 x FP16 -> BF16
+    assert call(at=B.ACL_FLOAT) == ns                      # FP32 is not a grouped-Ex type
+    assert call(at=B.ACL_BF16, ct=B.ACL_BF16, comp=B.COMPUTE["COMPUTE_16F"]) == ns
+    assert call(gc=-1) == inv
+    assert call(ta=(7,)) == inv
+    assert call(m=(-1,)) == inv
+    assert call(size=(-2,)) == inv
+    assert call(lda=(15,)) == inv
+    assert call(ldb=(15,)) == inv
+    assert call(ldc=(15,)) == inv
+    assert call(A=None) == inv
+    assert call(A=None, size=(0,)) == 0                    # no problems: arrays may be NULL, nothing runs
+    assert call(gc=0) == 0
+
+
 # ------------------------------------------------------------------ GEMM rows
 def _gemm_degenerate(r):
     m, n = int(r["m"]), int(r["n"])
