@@ -299,6 +299,21 @@ def bench_gemm(torch, B, W, h, timer):
     ms = timer(f, 5)
     tf = flops / (ms * 1e-3) / 1e12
     out["grouped_fp32_64"] = {"ms": ms, "tflops": tf, "frac_of_peak": tf / FP32_PEAK_TFLOPS, "flop": flops}
+    # the same 64 shapes in BF16 (BF16 C) through aclblasGemmGroupedBatchedEx
+    del As, Bs, Cs
+    Ah = [torch.randn(int(mm * kk), device=dev).to(torch.bfloat16) for mm, nn, kk in mnk]
+    Bh = [torch.randn(int(kk * nn), device=dev).to(torch.bfloat16) for mm, nn, kk in mnk]
+    Ch = [torch.randn(int(mm * nn), device=dev).to(torch.bfloat16) for mm, nn, kk in mnk]
+    ptr = lambda ts: torch.tensor([x.data_ptr() for x in ts], dtype=torch.int64, device=dev)
+    pa, pb, pc = ptr(Ah), ptr(Bh), ptr(Ch)
+    f = lambda: B.gemm_grouped_batched_ex(h.value, [B.OP_N] * 64, [B.OP_N] * 64, g[0], g[1], g[2], [1.0] * 64,
+                                          pa.data_ptr(), B.ACL_BF16, g[0], pb.data_ptr(), B.ACL_BF16, g[2],
+                                          [0.5] * 64, pc.data_ptr(), B.ACL_BF16, g[0], [1] * 64,
+                                          B.COMPUTE["COMPUTE_32F"])
+    assert f() == 0
+    ms = timer(f, 5)
+    tf = flops / (ms * 1e-3) / 1e12
+    out["grouped_bf16_ex_64"] = {"ms": ms, "tflops": tf, "frac_of_peak": tf / BF16_PEAK_TFLOPS, "flop": flops}
     return out
 
 

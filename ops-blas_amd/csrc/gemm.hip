@@ -10,10 +10,6 @@
 // writes op(A)op(B) to a workspace temporary and runs a second alpha/beta pass
 // whenever alpha != 1 or beta != 0; here alpha and beta are applied in the
 // MFMA kernel's epilogue, so no workspace is needed and C is read at most once.
-#include <cmath>
-#include <cstring>
-#include <limits>
-
 #include "gemm_kernel.h"
 #include "drivers.h"
 #include "internal.h"
@@ -59,9 +55,7 @@ void LaunchScale(hipStream_t st, int outKind, void* C, int m, int n, int ldc, fl
 
 // C_b[:m, :n] = beta * C_b for every problem of a batch (pointer array).
 // Grid: perProblem blocks for each problem, problems along x (no 65535 limit).
-// PLUS_ZERO: the value stored is 0 + beta * C (a -0 product becomes +0), as
-// an epilogue that adds beta * C to a zero product does.
-template <int OK, bool PLUS_ZERO = false>
+template <int OK>
 __global__ __launch_bounds__(256) void ScaleCBatchedKernel(void* const* Carr, int m, int n, int ldc, float beta,
                                                             int perProblem)
 {
@@ -73,10 +67,7 @@ __global__ __launch_bounds__(256) void ScaleCBatchedKernel(void* const* Carr, in
         const int j = static_cast<int>(e / m);
         const int i = static_cast<int>(e - static_cast<long long>(j) * m);
         const size_t idx = static_cast<size_t>(j) * ldc + i;
-        float v = beta == 0.0f ? 0.0f : beta * LoadC<OK>(C, idx);
-        if (PLUS_ZERO) {
-            v = 0.0f + v;
-        }
+        const float v = beta == 0.0f ? 0.0f : beta * LoadC<OK>(C, idx);
         StoreC<OK>(C, idx, v);
     }
 }
@@ -100,23 +91,6 @@ void LaunchScaleBatched(hipStream_t st, int outKind, void* const* Carr, int m, i
         default:
             hipLaunchKernelGGL((ScaleCBatchedKernel<kOutF16>), grid, dim3(256), 0, st, Carr, m, n, ldc, beta, per);
             break;
-    }
-}
-
-// The beta-only pass of the grouped 16-bit / FP8 GEMM (0 + beta * C).
-void LaunchScaleGrouped(hipStream_t st, int outKind, void* const* Carr, int m, int n, int ldc, float beta, int batch)
-{
-    const long long total = static_cast<long long>(m) * n;
-    long long bx = (total + 255) / 256;
-    bx = bx > 64 ? 64 : bx;
-    const long long maxPer = 0x7FFFFFFFll / (batch > 0 ? batch : 1);
-    bx = bx > maxPer ? (maxPer > 0 ? maxPer : 1) : bx;
-    const int per = static_cast<int>(bx);
-    const dim3 grid(static_cast<unsigned>(bx * batch));
-    if (outKind == kOutBF16) {
-        hipLaunchKernelGGL((ScaleCBatchedKernel<kOutBF16, true>), grid, dim3(256), 0, st, Carr, m, n, ldc, beta, per);
-    } else {
-        hipLaunchKernelGGL((ScaleCBatchedKernel<kOutF16A, true>), grid, dim3(256), 0, st, Carr, m, n, ldc, beta, per);
     }
 }
 
@@ -205,16 +179,6 @@ aclblasStatus_t RunGemm(const _aclblas_handle* h, int ka, int kb, int ok, bool t
         LaunchGemmTyped<kF16, kF16, kOutF16>(st, p, batch, tile, group);
     } else if (ka == kF16 && kb == kF16 && ok == kOutF32) {
         LaunchGemmTyped<kF16, kF16, kOutF32>(st, p, batch, tile, group);
-    } else if (ka == kF16 && kb == kF16 && ok == kOutF16A) {
-        LaunchGemmTyped<kF16, kF16, kOutF16A>(st, p, batch, tile, group);
-    } else if (ok == kOutF16A && ka == kE4M3 && kb == kE4M3) {
-        LaunchGemmTyped<kE4M3, kE4M3, kOutF16A>(st, p, batch, tile, group);
-    } else if (ok == kOutF16A && ka == kE4M3 && kb == kE5M2) {
-        LaunchGemmTyped<kE4M3, kE5M2, kOutF16A>(st, p, batch, tile, group);
-    } else if (ok == kOutF16A && ka == kE5M2 && kb == kE4M3) {
-        LaunchGemmTyped<kE5M2, kE4M3, kOutF16A>(st, p, batch, tile, group);
-    } else if (ok == kOutF16A && ka == kE5M2 && kb == kE5M2) {
-        LaunchGemmTyped<kE5M2, kE5M2, kOutF16A>(st, p, batch, tile, group);
     } else if (ka == kE4M3 && kb == kE4M3) {
         LaunchGemmTyped<kE4M3, kE4M3, kOutF16>(st, p, batch, tile, group);
     } else if (ka == kE4M3 && kb == kE5M2) {
@@ -513,71 +477,6 @@ aclblasStatus_t RunBatched(aclblasHandle_t handle, aclDataType at, aclDataType b
                    lda, nullptr, ldb, beta, nullptr, ldc, nullptr, nullptr, A, B, C, batch);
 }
 
-// ------------------------------------------------------- grouped 16-bit / FP8
-// IEEE binary16 bits -> float (exact), for alpha / beta given under COMPUTE_16F.
-float HostHalfToFloat(uint16_t h)
-{
-    const uint32_t sign = static_cast<uint32_t>(h & 0x8000u) << 16;
-    const int exp = (h >> 10) & 0x1F;
-    const uint32_t frac = h & 0x3FFu;
-    float mag;
-    if (exp == 0) {
-        mag = std::ldexp(static_cast<float>(frac), -24);
-    } else if (exp == 31) {
-        mag = frac == 0 ? std::numeric_limits<float>::infinity() : std::numeric_limits<float>::quiet_NaN();
-    } else {
-        mag = std::ldexp(static_cast<float>(frac | 0x400u), exp - 25);
-    }
-    uint32_t bits;
-    std::memcpy(&bits, &mag, sizeof(bits));
-    bits |= sign;
-    float out;
-    std::memcpy(&out, &bits, sizeof(out));
-    return out;
-}
-
-bool IsFp8Type(aclDataType t) { return t == ACL_FLOAT8_E4M3FN || t == ACL_FLOAT8_E5M2; }
-
-// reference gemm_grouped_batched_ex_host.cpp:59-72
-bool GroupedExTypesSupported(aclDataType at, aclDataType bt, aclDataType ct, aclblasComputeType_t ctype)
-{
-    if (at == ACL_FLOAT16 && bt == ACL_FLOAT16 && ct == ACL_FLOAT16) {
-        return ctype == ACLBLAS_COMPUTE_16F || ctype == ACLBLAS_COMPUTE_32F;
-    }
-    if (at == ACL_BF16 && bt == ACL_BF16 && ct == ACL_BF16) {
-        return ctype == ACLBLAS_COMPUTE_32F;
-    }
-    return ct == ACL_FLOAT16 && ctype == ACLBLAS_COMPUTE_32F && IsFp8Type(at) && IsFp8Type(bt);
-}
-
-// reference gemm_grouped_batched_ex_host.cpp:115-175 (same checks, same order)
-aclblasStatus_t ValidateGroupedEx(const aclblasOperation_t* ta, const aclblasOperation_t* tb, const int* m,
-                                  const int* n, const int* k, const void* alpha, const int* lda, const int* ldb,
-                                  const void* beta, const int* ldc, int groupCount, const int* groupSize)
-{
-    if (groupCount < 0) {
-        return ACLBLAS_STATUS_INVALID_VALUE;
-    }
-    if (groupCount == 0) {
-        return ACLBLAS_STATUS_SUCCESS;
-    }
-    if (ta == nullptr || tb == nullptr || m == nullptr || n == nullptr || k == nullptr || alpha == nullptr ||
-        lda == nullptr || ldb == nullptr || beta == nullptr || ldc == nullptr || groupSize == nullptr) {
-        return ACLBLAS_STATUS_INVALID_VALUE;
-    }
-    for (int g = 0; g < groupCount; ++g) {
-        if (!IsValidOp(ta[g]) || !IsValidOp(tb[g]) || m[g] < 0 || n[g] < 0 || k[g] < 0 || groupSize[g] < 0) {
-            return ACLBLAS_STATUS_INVALID_VALUE;
-        }
-        const int minLda = ta[g] == ACLBLAS_OP_N ? MaxInt(1, m[g]) : MaxInt(1, k[g]);
-        const int minLdb = tb[g] == ACLBLAS_OP_N ? MaxInt(1, k[g]) : MaxInt(1, n[g]);
-        if (lda[g] < minLda || ldb[g] < minLdb || ldc[g] < MaxInt(1, m[g])) {
-            return ACLBLAS_STATUS_INVALID_VALUE;
-        }
-    }
-    return ACLBLAS_STATUS_SUCCESS;
-}
-
 }  // namespace
 
 aclblasStatus_t SgemmBatchedLaunch(const _aclblas_handle* h, bool transA, bool transB, int m, int n, int k,
@@ -597,97 +496,6 @@ aclblasStatus_t SgemmBatchedLaunch(const _aclblas_handle* h, bool transA, bool t
 }
 
 }  // namespace aclblas
-
-// Reference blas/gemm_grouped_batched_ex/arch35/gemm_grouped_batched_ex_host.cpp
-// :356-414.  Each group is one batched launch of the fused GEMM over its slice
-// of the concatenated device pointer arrays: FP32 accumulation for every input
-// type, then alpha * AB (+ beta * C), rounded to BF16 to nearest even or to
-// FP16 to nearest with ties away from zero (reference
-// gemm_grouped_batched_ex_epilogue_kernel.cpp:88-148; no FP8 non-finite rule).
-// A group without products (k == 0 or alpha == 0) stores 0 + beta * C, or
-// leaves C alone when beta == 1.  Asynchronous on the handle's stream (the
-// reference synchronises before returning).
-extern "C" aclblasStatus_t aclblasGemmGroupedBatchedEx(
-    aclblasHandle_t handle, const aclblasOperation_t transaArray[], const aclblasOperation_t transbArray[],
-    const int mArray[], const int nArray[], const int kArray[], const void* alphaArray, const void* const Aarray[],
-    aclDataType Atype, const int ldaArray[], const void* const Barray[], aclDataType Btype, const int ldbArray[],
-    const void* betaArray, void* const Carray[], aclDataType Ctype, const int ldcArray[], int groupCount,
-    const int groupSize[], aclblasComputeType_t computeType)
-{
-    using namespace aclblas;
-    if (handle == nullptr) {
-        return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
-    }
-    if (!GroupedExTypesSupported(Atype, Btype, Ctype, computeType)) {
-        LogError("aclblasGemmGroupedBatchedEx", "unsupported type combination A=%d B=%d C=%d compute=%d",
-                 static_cast<int>(Atype), static_cast<int>(Btype), static_cast<int>(Ctype),
-                 static_cast<int>(computeType));
-        return ACLBLAS_STATUS_NOT_SUPPORTED;
-    }
-    aclblasStatus_t st = ValidateGroupedEx(transaArray, transbArray, mArray, nArray, kArray, alphaArray, ldaArray,
-                                           ldbArray, betaArray, ldcArray, groupCount, groupSize);
-    if (st != ACLBLAS_STATUS_SUCCESS || groupCount == 0) {
-        if (st != ACLBLAS_STATUS_SUCCESS) {
-            LogError("aclblasGemmGroupedBatchedEx", "invalid argument");
-        }
-        return st;
-    }
-    long long problems = 0;
-    for (int g = 0; g < groupCount; ++g) {
-        problems += groupSize[g];
-    }
-    if (problems > 0 && (Aarray == nullptr || Barray == nullptr || Carray == nullptr)) {
-        LogError("aclblasGemmGroupedBatchedEx", "matrix pointer arrays must not be NULL");
-        return ACLBLAS_STATUS_INVALID_VALUE;
-    }
-    if (problems > 0x7FFFFFFFll) {
-        return ACLBLAS_STATUS_INVALID_VALUE;
-    }
-    const bool half = computeType == ACLBLAS_COMPUTE_16F;
-    auto scalar = [&](const void* arr, int g) {
-        return half ? HostHalfToFloat(static_cast<const uint16_t*>(arr)[g]) : static_cast<const float*>(arr)[g];
-    };
-    // nothing to launch (no problem, or only beta == 1 groups without products):
-    // success before any device work, as the reference (host.cpp:393-395)
-    bool work = false;
-    for (int g = 0; g < groupCount && !work; ++g) {
-        if (groupSize[g] > 0 && mArray[g] > 0 && nArray[g] > 0) {
-            work = (kArray[g] > 0 && scalar(alphaArray, g) != 0.0f) || scalar(betaArray, g) != 1.0f;
-        }
-    }
-    if (!work) {
-        return ACLBLAS_STATUS_SUCCESS;
-    }
-    if (!HandleHasDevice(handle)) {
-        return ACLBLAS_STATUS_EXECUTION_FAILED;
-    }
-    const int ok = Ctype == ACL_BF16 ? kOutBF16 : kOutF16A;
-    long long start = 0;
-    for (int g = 0; g < groupCount; ++g) {
-        const int size = groupSize[g];
-        const int m = mArray[g];
-        const int n = nArray[g];
-        const int k = kArray[g];
-        if (size > 0 && m > 0 && n > 0) {
-            const float alpha = scalar(alphaArray, g);
-            const float beta = scalar(betaArray, g);
-            if (k > 0 && alpha != 0.0f) {
-                st = RunGemm(handle, KindOf(Atype), KindOf(Btype), ok, transaArray[g] != ACLBLAS_OP_N,
-                             transbArray[g] != ACLBLAS_OP_N, m, n, k, alpha, nullptr, ldaArray[g], nullptr,
-                             ldbArray[g], beta, nullptr, ldcArray[g], nullptr, nullptr, Aarray + start,
-                             Barray + start, Carray + start, size);
-            } else if (beta != 1.0f) {
-                LaunchScaleGrouped(handle->stream, ok, Carray + start, m, n, ldcArray[g], beta, size);
-                st = LaunchStatus("aclblasGemmGroupedBatchedEx");
-            }
-            if (st != ACLBLAS_STATUS_SUCCESS) {
-                return st;
-            }
-        }
-        start += size;
-    }
-    return ACLBLAS_STATUS_SUCCESS;
-}
 
 extern "C" aclblasStatus_t aclblasGemmBatchedEx(
     aclblasHandle_t handle, aclblasOperation_t transa, aclblasOperation_t transb, int m, int n, int k,

@@ -831,6 +831,7 @@ struct GroupedProblem {
     int tilesM, tilesN;
     int tileStart;          // first global tile index of this problem
     int vecA, vecB, vecC;
+    int slot;               // index into the device pointer arrays (when the launch passes them)
 };
 
 // L2 group height of the grouped kernel's tile walk (problems are small: a
@@ -841,9 +842,15 @@ constexpr int kGroupedL2Rows = 8;
 // of all problems (problems ordered by decreasing k on the host, so the
 // longest tiles start first); each workgroup finds its problem by binary
 // search over tileStart.
+// With device pointer arrays (Aarr != nullptr) a record's A, B and C come from
+// Aarr/Barr/Carr[slot], and its vector flags hold only the leading-dimension
+// part of the alignment test: the pointers are checked here.
 template <int KA, int KB, int OK, int BM, int BN>
 __global__ __launch_bounds__(kThreads) void GroupedGemmKernel(const GroupedProblem* __restrict__ probs,
-                                                             int numProbs, int totalTiles)
+                                                             int numProbs, int totalTiles,
+                                                             const void* const* __restrict__ Aarr,
+                                                             const void* const* __restrict__ Barr,
+                                                             void* const* __restrict__ Carr)
 {
     __shared__ __attribute__((aligned(16))) uint8_t sA[2][BM * kRowBytes];
     __shared__ __attribute__((aligned(16))) uint8_t sB[2][BN * kRowBytes];
@@ -876,6 +883,14 @@ __global__ __launch_bounds__(kThreads) void GroupedGemmKernel(const GroupedProbl
         q.vecA = g.vecA != 0;
         q.vecB = g.vecB != 0;
         q.vecC = g.vecC != 0;
+        if (Aarr != nullptr) {
+            q.A = static_cast<const uint8_t*>(Aarr[g.slot]);
+            q.B = static_cast<const uint8_t*>(Barr[g.slot]);
+            q.C = Carr[g.slot];
+            q.vecA = q.vecA && (reinterpret_cast<uintptr_t>(q.A) % 16 == 0);
+            q.vecB = q.vecB && (reinterpret_cast<uintptr_t>(q.B) % 16 == 0);
+            q.vecC = q.vecC && (reinterpret_cast<uintptr_t>(q.C) % 16 == 0);
+        }
         q.alpha = g.alpha;
         q.beta = g.beta;
         GemmTile<KA, KB, OK, BM, BN>(q, tm * BM, tn * BN, sA, sB);
