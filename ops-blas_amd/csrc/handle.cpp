@@ -84,7 +84,8 @@ void Emit(int level, const char* api, const char* fmt, va_list ap)
     }
 }
 
-aclblasStatus_t AllocateLibraryWorkspace(_aclblas_handle* h, size_t size)
+// The handle's own device arena (32 MiB at creation, grown on demand).
+aclblasStatus_t AcquireOwnedArena(_aclblas_handle* h, size_t size)
 {
     if (size == 0) {
         return ACLBLAS_STATUS_INVALID_VALUE;
@@ -96,24 +97,34 @@ aclblasStatus_t AllocateLibraryWorkspace(_aclblas_handle* h, size_t size)
     }
     h->workspace = p;
     h->workspace_size = size;
-    h->library_workspace_size = size;
+    h->owned_size_hint = size;
     h->owner = WorkspaceOwner::Library;
+    h->arena_in_graph = false;
     return ACLBLAS_STATUS_SUCCESS;
 }
 
-void FreeLibraryWorkspace(_aclblas_handle* h)
+// Gives up the handle's own arena.  An arena that a graph captured on this
+// handle may still address is parked until aclblasDestroy instead of freed, so
+// replaying such a graph after the arena was replaced never touches freed
+// memory.
+void ReleaseOwnedArena(_aclblas_handle* h)
 {
     if (h->owner != WorkspaceOwner::Library || h->workspace == nullptr) {
         return;
     }
-    (void)hipFree(h->workspace);
+    if (h->arena_in_graph) {
+        h->parked_arenas.push_back(h->workspace);
+    } else {
+        (void)hipFree(h->workspace);
+    }
     h->workspace = nullptr;
     h->workspace_size = 0;
+    h->arena_in_graph = false;
 }
 
-// Back to a library-owned workspace after a user workspace or a stream change
-// (reference aclblas_handle_internal.h:133-154).
-aclblasStatus_t ResetToLibraryWorkspace(_aclblas_handle* h)
+// After a borrowed workspace or a stream change the handle works in an arena
+// of its own again, of the size it last had.
+aclblasStatus_t ReturnToOwnedArena(_aclblas_handle* h)
 {
     if (!HandleHasDevice(h)) {
         h->workspace = nullptr;
@@ -131,8 +142,8 @@ aclblasStatus_t ResetToLibraryWorkspace(_aclblas_handle* h)
     h->workspace = nullptr;
     h->workspace_size = 0;
     h->owner = WorkspaceOwner::Library;
-    size_t size = h->library_workspace_size > 0 ? h->library_workspace_size : kDefaultWorkspaceSize;
-    return AllocateLibraryWorkspace(h, size);
+    size_t size = h->owned_size_hint > 0 ? h->owned_size_hint : kDefaultWorkspaceSize;
+    return AcquireOwnedArena(h, size);
 }
 
 }  // namespace
@@ -170,22 +181,26 @@ aclblasStatus_t EnsureWorkspace(_aclblas_handle* h, size_t bytes)
     if (!HandleHasDevice(h)) {
         return ACLBLAS_STATUS_EXECUTION_FAILED;
     }
+    const bool capturing = StreamCapturing(h);
     if (h->owner == WorkspaceOwner::User) {
         if (bytes > h->workspace_size) {
-            LogError("aclblasHandle", "user workspace too small: required=%zu available=%zu", bytes,
-                     h->workspace_size);
+            LogError("aclblasHandle", "the caller's workspace holds %zu bytes, this call needs %zu",
+                     h->workspace_size, bytes);
             return ACLBLAS_STATUS_ALLOC_FAILED;
         }
         return ACLBLAS_STATUS_SUCCESS;
     }
     if (bytes <= h->workspace_size) {
+        if (capturing) {
+            h->arena_in_graph = true;
+        }
         return ACLBLAS_STATUS_SUCCESS;
     }
     if (bytes > kMaxWorkspaceSize) {
         LogError("aclblasHandle", "workspace of %zu bytes exceeds the %zu byte limit", bytes, kMaxWorkspaceSize);
         return ACLBLAS_STATUS_ALLOC_FAILED;
     }
-    if (StreamCapturing(h)) {
+    if (capturing) {
         LogError("aclblasHandle", "workspace must grow to %zu bytes, which cannot happen during stream capture: "
                  "run the call once before capturing or set a workspace", bytes);
         return ACLBLAS_STATUS_NOT_SUPPORTED;
@@ -199,8 +214,15 @@ aclblasStatus_t EnsureWorkspace(_aclblas_handle* h, size_t bytes)
     if (size > kMaxWorkspaceSize) {
         size = kMaxWorkspaceSize;
     }
-    FreeLibraryWorkspace(h);
-    return AllocateLibraryWorkspace(h, size);
+    ReleaseOwnedArena(h);
+    return AcquireOwnedArena(h, size);
+}
+
+void MarkWorkspaceUse(_aclblas_handle* h)
+{
+    if (h->owner == WorkspaceOwner::Library && h->workspace != nullptr && StreamCapturing(h)) {
+        h->arena_in_graph = true;
+    }
 }
 
 bool StreamCapturing(const _aclblas_handle* h)
@@ -301,7 +323,7 @@ aclblasStatus_t aclblasCreate(aclblasHandle_t* handle)
         cus = 256;
     }
     h->cu_count = cus;
-    aclblasStatus_t st = AllocateLibraryWorkspace(h, kDefaultWorkspaceSize);
+    aclblasStatus_t st = AcquireOwnedArena(h, kDefaultWorkspaceSize);
     if (st != ACLBLAS_STATUS_SUCCESS) {
         delete h;
         return st;
@@ -319,7 +341,11 @@ aclblasStatus_t aclblasDestroy(aclblasHandle_t handle)
         LogError("aclblasDestroy", "stream synchronisation failed before handle destruction");
         return ACLBLAS_STATUS_EXECUTION_FAILED;
     }
-    FreeLibraryWorkspace(handle);
+    handle->arena_in_graph = false;
+    ReleaseOwnedArena(handle);
+    for (void* p : handle->parked_arenas) {
+        (void)hipFree(p);
+    }
     if (handle->host_stage != nullptr) {
         (void)hipHostFree(handle->host_stage);
     }
@@ -343,7 +369,7 @@ aclblasStatus_t aclblasSetStream(aclblasHandle_t handle, aclrtStream stream)
         return st;
     }
     handle->stream = reinterpret_cast<hipStream_t>(stream);
-    return ResetToLibraryWorkspace(handle);
+    return ReturnToOwnedArena(handle);
 }
 
 aclblasStatus_t aclblasGetStream(aclblasHandle_t handle, aclrtStream* stream)
@@ -374,12 +400,27 @@ aclblasStatus_t aclblasSetWorkspace(aclblasHandle_t handle, void* workspace, siz
         return st;
     }
     if (handle->owner == WorkspaceOwner::Library && handle->workspace != nullptr) {
-        handle->library_workspace_size = handle->workspace_size;
-        FreeLibraryWorkspace(handle);
+        handle->owned_size_hint = handle->workspace_size;
+        ReleaseOwnedArena(handle);
     }
     handle->workspace = workspace;
     handle->workspace_size = workspaceSize;
     handle->owner = WorkspaceOwner::User;
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+aclblasStatus_t aclblasSetLaunchEvents(aclblasHandle_t handle, void* const events[], int count)
+{
+    if (handle == nullptr) {
+        return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    }
+    if (count < 0 || (count > 0 && events == nullptr)) {
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    handle->launch_events.assign(static_cast<size_t>(count), nullptr);
+    for (int i = 0; i < count; ++i) {
+        handle->launch_events[static_cast<size_t>(i)] = static_cast<hipEvent_t>(events[i]);
+    }
     return ACLBLAS_STATUS_SUCCESS;
 }
 

@@ -35,7 +35,14 @@ struct _aclblas_handle {
     void* workspace = nullptr;
     size_t workspace_size = 0;
     aclblas::WorkspaceOwner owner = aclblas::WorkspaceOwner::Library;
-    size_t library_workspace_size = 0;
+    // Size of the handle's own arena to restore after a borrowed workspace.
+    size_t owned_size_hint = 0;
+    // Set when a call captured into a graph used the handle's own arena; such
+    // an arena is parked (freed by aclblasDestroy) rather than freed when it is
+    // replaced.  Graphs captured on a handle must not be replayed after
+    // aclblasDestroy, nor after a borrowed workspace they used is freed.
+    bool arena_in_graph = false;
+    std::vector<void*> parked_arenas;
     // Device the handle was created on and its compute-unit count (used to size
     // persistent grids).  device < 0 means no GPU was visible at creation.
     int device = -1;
@@ -49,6 +56,9 @@ struct _aclblas_handle {
     // Pinned copies of tables staged while the stream was being captured into
     // a graph (each replay reads its own copy); freed with the handle.
     std::vector<void*> captured_stages;
+    // Optional events recorded around the kernels of multi-launch calls
+    // (aclblasSetLaunchEvents): event 2i before and 2i+1 after launch i.
+    std::vector<hipEvent_t> launch_events;
     // Per-operation forced launch choice {set, a, b} (aclblasSetTuningOverride);
     // set == 0 means the measured table decides.
     int tune_override[4][3] = {};
@@ -64,10 +74,15 @@ void LogInfo(const char* api, const char* fmt, ...) __attribute__((format(printf
 // Synchronises the handle's stream; EXECUTION_FAILED on error.
 aclblasStatus_t SyncHandleStream(_aclblas_handle* h);
 
-// Grows the library workspace to at least `bytes` (reference
-// aclblas_handle_internal.h:162-209).  Never called on the hot path: every
-// hot-path kernel here runs without a workspace.
+// Makes the active workspace at least `bytes` long, growing the handle's own
+// arena by doubling when needed (reference aclblas_handle_internal.h:162-209);
+// a borrowed workspace that is too small gives ALLOC_FAILED.  Every call that
+// addresses the workspace goes through here or through MarkWorkspaceUse.
 aclblasStatus_t EnsureWorkspace(_aclblas_handle* h, size_t bytes);
+
+// Records that the current call addresses the active workspace as it is (no
+// growth); inside a stream capture this keeps the arena alive for the graph.
+void MarkWorkspaceUse(_aclblas_handle* h);
 
 // True while the handle's stream is being captured into a graph.
 bool StreamCapturing(const _aclblas_handle* h);
@@ -77,6 +92,15 @@ bool StreamCapturing(const _aclblas_handle* h);
 // buffer; *dev receives the device address.  Blocks only while a previous
 // staging copy of this handle is still pending.
 aclblasStatus_t StageToWorkspace(_aclblas_handle* h, const void* src, size_t bytes, void** dev);
+
+// Records the handle's launch event `slot` (aclblasSetLaunchEvents) on its
+// stream, if the caller registered one.
+inline void MarkLaunch(_aclblas_handle* h, size_t slot)
+{
+    if (slot < h->launch_events.size() && h->launch_events[slot] != nullptr) {
+        (void)hipEventRecord(h->launch_events[slot], h->stream);
+    }
+}
 
 // Checks that the handle can launch work and that the last launch was accepted.
 inline bool HandleHasDevice(const _aclblas_handle* h) { return h->device >= 0; }

@@ -81,6 +81,7 @@ extern "C" aclblasStatus_t aclblasSmatinvBatched(
                  handle->workspace == nullptr ? static_cast<size_t>(0) : handle->workspace_size);
         return ACLBLAS_STATUS_EXECUTION_FAILED;
     }
+    MarkWorkspaceUse(handle);
     auto* ws = static_cast<unsigned char*>(handle->workspace);
     float** table = reinterpret_cast<float**>(ws);
     float* blocks = reinterpret_cast<float*>(ws + tableBytes);

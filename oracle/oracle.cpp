@@ -614,6 +614,60 @@ double orc_time_getrf_getrs(const float* A, const float* B, int n, int nrhs, int
     return times.empty() ? 0.0 : times[times.size() / 2];
 }
 
+// Wall time of `reps` runs of getrf + getrs (trans N) over a batch of systems
+// of mixed orders: system i is the n[i] x n[i] column-major matrix at
+// A + offA[i] with nrhs[i] right-hand sides at B + offB[i] (leading dimension
+// n[i]).  Systems are handed to the threads largest first.  Returns seconds
+// per run (median); inputs are left untouched.
+double orc_time_getrf_getrs_mixed(const float* A, const long long* offA, const float* B, const long long* offB,
+                                  const int* n, const int* nrhs, int count, int threads, int reps)
+{
+    std::vector<int> order(static_cast<size_t>(count));
+    for (int i = 0; i < count; ++i) order[static_cast<size_t>(i)] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n[a] > n[b]; });
+    size_t atotal = 0, btotal = 0;
+    std::vector<size_t> wa(static_cast<size_t>(count)), wb(static_cast<size_t>(count));
+    for (int i = 0; i < count; ++i) {
+        wa[static_cast<size_t>(i)] = atotal;
+        wb[static_cast<size_t>(i)] = btotal;
+        atotal += static_cast<size_t>(n[i]) * n[i];
+        btotal += static_cast<size_t>(n[i]) * nrhs[i];
+    }
+    std::vector<float> a(atotal), x(btotal);
+    std::vector<double> times;
+    for (int r = 0; r < reps + 1; ++r) {
+        for (int i = 0; i < count; ++i) {
+            std::memcpy(a.data() + wa[static_cast<size_t>(i)], A + offA[i],
+                        sizeof(float) * static_cast<size_t>(n[i]) * n[i]);
+            std::memcpy(x.data() + wb[static_cast<size_t>(i)], B + offB[i],
+                        sizeof(float) * static_cast<size_t>(n[i]) * nrhs[i]);
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        ParallelFor(count, threads, [&](int q) {
+            const int i = order[static_cast<size_t>(q)];
+            const int ni = n[i];
+            float* Ab = a.data() + wa[static_cast<size_t>(i)];
+            std::vector<int> piv(static_cast<size_t>(ni));
+            GetrfOne(Ab, ni, ni, piv.data());
+            std::vector<int> perm;
+            BuildPerm(piv.data(), ni, perm);
+            std::vector<float> v(static_cast<size_t>(ni));
+            for (int c = 0; c < nrhs[i]; ++c) {
+                float* col = x.data() + wb[static_cast<size_t>(i)] + static_cast<size_t>(c) * ni;
+                for (int k = 0; k < ni; ++k) v[k] = col[perm[k]];
+                SolveN(Ab, ni, ni, v.data());
+                for (int k = 0; k < ni; ++k) col[k] = v[k];
+            }
+        });
+        const auto t1 = std::chrono::steady_clock::now();
+        if (r > 0) {
+            times.push_back(std::chrono::duration<double>(t1 - t0).count());
+        }
+    }
+    std::sort(times.begin(), times.end());
+    return times.empty() ? 0.0 : times[times.size() / 2];
+}
+
 int orc_hardware_threads() { return static_cast<int>(std::thread::hardware_concurrency()); }
 
 // ---------------------------------------------------------------------------

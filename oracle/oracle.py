@@ -21,6 +21,7 @@ _lib = None
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
@@ -57,6 +58,9 @@ def lib():
                                         c.c_int, _i32p, c.c_int, c.c_int]
         L.orc_time_getrf_getrs.argtypes = [_f32p, _f32p, c.c_int, c.c_int, c.c_int, c.c_int, c.c_int]
         L.orc_time_getrf_getrs.restype = c.c_double
+        L.orc_time_getrf_getrs_mixed.argtypes = [_f32p, _i64p, _f32p, _i64p, _i32p, _i32p, c.c_int, c.c_int,
+                                                 c.c_int]
+        L.orc_time_getrf_getrs_mixed.restype = c.c_double
         L.orc_hardware_threads.restype = c.c_int
         L.orc_gemm.argtypes = [c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_double, c.c_void_p, c.c_int,
                                c.c_int, c.c_void_p, c.c_int, c.c_int, c.c_double, c.c_void_p, c.c_int, c.c_int,
@@ -188,6 +192,18 @@ def time_getrf_getrs(A, B, nthreads, reps=3):
     batch, n, _ = A.shape
     nrhs = B.shape[1]
     return float(lib().orc_time_getrf_getrs(A.reshape(-1), B.reshape(-1), n, nrhs, batch, nthreads, reps))
+
+
+def time_getrf_getrs_mixed(A, offA, B, offB, n, nrhs, nthreads, reps=3):
+    """Median seconds of one getrf+getrs pass over mixed-order systems packed in the flat float32 arrays A and
+    B (system i at offA[i] / offB[i], column-major, leading dimension n[i])."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    offA = np.ascontiguousarray(offA, dtype=np.int64)
+    offB = np.ascontiguousarray(offB, dtype=np.int64)
+    n = np.ascontiguousarray(n, dtype=np.int32)
+    nrhs = np.ascontiguousarray(nrhs, dtype=np.int32)
+    return float(lib().orc_time_getrf_getrs_mixed(A, offA, B, offB, n, nrhs, len(n), nthreads, reps))
 
 
 # ---------------------------------------------------------------- GEMM

@@ -157,22 +157,42 @@ def _ulp(x, ct):
     return np.ldexp(1.0, (np.maximum(e, emin) - mant).astype(np.int64))
 
 
-def check_gemm(out, gold, ct, ref, threshold, mult=10.0):
-    """Reference MERE/MARE; if it fails, the FP32 dot-product error bound element by element."""
+# Rows of the reference's own tables that miss its MERE/MARE criterion and pass on the element-wise FP32
+# accumulation bound instead, with the reason for each (case id -> reason).  Any other row of those tables
+# must meet MERE/MARE.
+FALLBACK_ALLOWED = {}
+
+
+def check_gemm(out, gold, ct, ref, threshold, mult=10.0, case=None):
+    """Reference MERE/MARE.  For a row of the reference's tables (`case` names it) that criterion is the bar,
+    except for the rows pinned in FALLBACK_ALLOWED; for this build's own shapes (`case` None) the element-wise
+    FP32-accumulation bound is the bar when MERE/MARE misses."""
     ok, detail = O.mere_mare(out, gold, threshold, mult)
     if ok:
         return
+    if case is not None and case not in FALLBACK_ALLOWED:
+        fallback_ok = _within_accumulation_bound(out, gold, ct, ref)[0]
+        raise AssertionError("%s misses the reference criterion (%s); element-wise FP32 bound %s" %
+                             (case, detail, "met" if fallback_ok else "exceeded"))
+    ok2, worst, special_ok = _within_accumulation_bound(out, gold, ct, ref)
+    assert special_ok, detail
+    assert ok2, "%s; FP32-accumulation bound exceeded by x%.3g" % (detail, worst)
+
+
+def _within_accumulation_bound(out, gold, ct, ref):
+    """|out - gold| <= gamma_(k+2) (|alpha| |A||B| + |beta| |C|) + ulp(gold) element by element, and the same
+    non-finite pattern.  Returns (pass, worst ratio, non-finite pattern equal)."""
     k = ref[5]
     o = out.astype(np.float64)
     g = gold.astype(np.float64)
     special = ~np.isfinite(g) | ~np.isfinite(o)
-    assert np.array_equal(np.isnan(o), np.isnan(g)) and np.array_equal(o[np.isinf(o)], g[np.isinf(o)]), detail
+    special_ok = bool(np.array_equal(np.isnan(o), np.isnan(g)) and np.array_equal(o[np.isinf(o)], g[np.isinf(o)]))
     u = 2.0 ** -24
     gamma = (k + 2) * u / (1 - (k + 2) * u)
     bound = gamma * _abs_terms(*ref) + _ulp(gold, ct)
     err = np.where(special, 0.0, np.abs(o - g))
     worst = float(np.max(err / bound)) if err.size else 0.0
-    assert worst <= 1.0, "%s; FP32-accumulation bound exceeded by x%.3g" % (detail, worst)
+    return special_ok and worst <= 1.0, worst, special_ok
 
 
 def _as_f32(raw, ct):
@@ -223,7 +243,7 @@ def test_sgemm_ex_reference_cases(handle, row):
     if res is None:
         return _null_c_call(handle, row, "sgemm_ex")
     out, gold, ct, ref = res
-    check_gemm(out, gold, ct, ref, MERE[ct])
+    check_gemm(out, gold, ct, ref, MERE[ct], case="sgemm_ex:" + row["case_name"])
 
 
 GEMMEX = [r for r in load_cases("gemm_ex.csv") if status_code(r["expect_result"]) == 0
@@ -240,7 +260,7 @@ def test_gemm_ex_reference_cases(handle, row):
     mult = 10.0
     if ct == "FP16" and row["computeType"] == "COMPUTE_16F" and k > 160:
         mult = max(10.0, k / 3.0)
-    check_gemm(out, gold, ct, ref, MERE[ct], mult)
+    check_gemm(out, gold, ct, ref, MERE[ct], mult, case="gemm_ex:" + row["case_name"])
 
 
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
@@ -330,7 +350,7 @@ def test_gemm_batched_ex_reference_cases(handle, row):
         out = _as_f32(to_host(Cs[b]), ct).reshape(n, ldc)[:, :m]
         gold = _as_f32(golds[b], ct).reshape(n, ldc)[:, :m]
         try:
-            check_gemm(out, gold, ct, refs[b], MERE[ct], mult)
+            check_gemm(out, gold, ct, refs[b], MERE[ct], mult, case="gemm_batched_ex:%s#%d" % (row["case_name"], b))
         except AssertionError as e:
             raise AssertionError("batch %d: %s" % (b, e))
 

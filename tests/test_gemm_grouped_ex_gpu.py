@@ -106,7 +106,8 @@ def test_grouped_ex_reference_cases(handle, row):
         # rows below m of each column are untouched
         assert np.array_equal(out_raw.reshape(n, ldc)[:, m:], cq.reshape(n, ldc)[:, m:])
         try:
-            check_gemm(out, gf, ct, ref, thr if thr > 0 else 2.0 ** -10, mult)
+            check_gemm(out, gf, ct, ref, thr if thr > 0 else 2.0 ** -10, mult,
+                       case="gemm_grouped_batched_ex:%s#%d" % (row["case_name"], i))
         except AssertionError as e:
             raise AssertionError("group %d problem %d: %s" % (g, i, e))
 

@@ -89,7 +89,7 @@ def test_grouped_reference_cases(handle, row):
     assert st == 0
     thr = float(row["mere_threshold"] or 2.0 ** -13)
     mult = float(row["mare_multiplier"] or 10)
-    for p in probs:
+    for idx, p in enumerate(probs):
         g = p["g"]
         m, n, k = g["m"], g["n"], g["k"]
         out = to_host(p["dC"])
@@ -105,7 +105,7 @@ def test_grouped_reference_cases(handle, row):
         if k == 0 or g["alpha"] == 0.0:
             assert np.array_equal(o, gg)
         else:
-            check_gemm(o, gg, "FP32", ref, thr, mult)
+            check_gemm(o, gg, "FP32", ref, thr, mult, case="gemm_grouped_batched:%s#%d" % (row["case_name"], idx))
         # rows between m and ldc are untouched
         assert np.array_equal(out.reshape(n, g["ldc"])[:, m:], p["c"].reshape(n, g["ldc"])[:, m:])
 
