@@ -198,6 +198,18 @@ aclblasStatus_t aclblasSgetriBatched(
     aclblasHandle_t handle, int n, const float* const Aarray[], int lda, const int* PivotArray,
     float* const Carray[], int ldc, int* infoArray, int batchSize);
 
+/* reference include/cann_ops_blas.h:339-341.  B := alpha op(A)^-1 B (side
+ * LEFT, A m x m) or alpha B op(A)^-1 (side RIGHT, A n x n), A triangular
+ * (uplo; diag UNIT: the diagonal is taken as 1 and not read), op = N, T or C
+ * (= T), column-major FP32, B m x n (ldb).  Argument checks and their order as
+ * the reference's strsm_host.cpp:406-452 (every bad value, including an
+ * invalid side, uplo, trans or diag, gives INVALID_VALUE); m == 0 or n == 0
+ * returns before B and A are checked; alpha == 0 writes zeros to B without
+ * reading A.  No workspace. */
+aclblasStatus_t aclblasStrsm(
+    aclblasHandle_t handle, aclblasSideMode_t side, aclblasFillMode_t uplo, aclblasOperation_t trans,
+    aclblasDiagType_t diag, int m, int n, const float* alpha, const float* A, int lda, float* B, int ldb);
+
 /* reference include/cann_ops_blas.h:434-436.  Ainv[i] = A[i]^-1 for
  * 0 <= n <= 32 (pivoted LU, then the out-of-place inverse; A and Ainv are
  * device arrays of device pointers, column-major, Ainv[i] must not overlap
@@ -232,25 +244,59 @@ aclblasStatus_t aclblasSgetrsBatchedAdjoint(
 /* Mixed-precision batched solve                                             */
 /* ------------------------------------------------------------------------ */
 
+/* Precision of the LU's trailing updates A22 += (-L21) U12 (no reference
+ * counterpart).  Panels, pivots, the diagonal blocks and every accumulation are
+ * FP32 in all modes. */
+typedef enum {
+    ACLBLAS_MIXED_UPDATE_FP32 = 0,   /* FP32 matrix instruction */
+    ACLBLAS_MIXED_UPDATE_BF16 = 1,   /* operands rounded to BF16: one product */
+    ACLBLAS_MIXED_UPDATE_BF16X2 = 2, /* operands split into two BF16 planes: three products */
+    ACLBLAS_MIXED_UPDATE_BF16X3 = 3  /* three BF16 planes (the FP32 value): six products */
+} aclblasMixedUpdate_t;
+
 /* Solves A_i X_i = B_i for a batch of systems of different orders (no
  * counterpart in the reference; it follows the reference's LU semantics of
  * blas/getrf_batched/arch35/sgetrf_batched_kernel.cpp:52-150 - partial
  * pivoting, stop at an exact zero pivot - with a blocked algorithm).  Per
  * system: n = nArray[i] <= 256, nrhs = nrhsArray[i] <= 16 (larger:
  * NOT_SUPPORTED), column-major FP32 A (lda), B (ldb), X (ldx); A and B are
- * not modified.  The LU runs in FP32 with its trailing updates on BF16 matrix
- * cores (operands split into three BF16 terms); the solution is refined with
- * FP64 residuals until every right-hand side has normwise backward error
- * |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol, or maxIter corrections.
+ * not modified.  The LU runs in FP32 with its trailing updates on matrix cores
+ * in the precision `update`; the solution is refined with FP64 residuals until
+ * every right-hand side has normwise backward error
+ * |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol, the backward error fails
+ * to halve in a correction, or maxIter corrections have been applied.  With a
+ * BF16 mode, a system that does not reach tol (or meets a zero pivot) is
+ * factored again with FP32 updates and solved again (fallbackArray[i] = 1).
  * nArray .. ldxArray and the pointer arrays are HOST arrays (device matrices);
- * iterArray (corrections applied), berrArray (final worst backward error) and
- * infoArray (0, or k+1 for an exact zero pivot at column k, X = NaN) are
- * device arrays of batchCount entries, each may be NULL.  Works in the
- * handle's workspace (4 n^2 + O(n^2/8) bytes per system plus a 48-byte record). */
+ * iterArray (corrections applied by the final factorisation), berrArray (final
+ * worst backward error), infoArray (0, or k+1 for an exact zero pivot at
+ * column k of the final factorisation, X = NaN) and fallbackArray are device
+ * arrays of batchCount entries, each may be NULL.  INVALID_ENUM for an unknown
+ * `update`.  Works in the handle's workspace (about 4.1 n^2 + 128 n bytes per
+ * system plus an 80-byte record). */
+aclblasStatus_t aclblasSgesvBatchedMixedEx(
+    aclblasHandle_t handle, int batchCount, const int* nArray, const int* nrhsArray, const float* const Aarray[],
+    const int* ldaArray, const float* const Barray[], const int* ldbArray, float* const Xarray[],
+    const int* ldxArray, double tol, int maxIter, aclblasMixedUpdate_t update, int* iterArray, float* berrArray,
+    int* infoArray, int* fallbackArray);
+
+/* aclblasSgesvBatchedMixedEx with update = ACLBLAS_MIXED_UPDATE_BF16X2 and no
+ * fallback array. */
 aclblasStatus_t aclblasSgesvBatchedMixed(
     aclblasHandle_t handle, int batchCount, const int* nArray, const int* nrhsArray, const float* const Aarray[],
     const int* ldaArray, const float* const Barray[], const int* ldbArray, float* const Xarray[],
     const int* ldxArray, double tol, int maxIter, int* iterArray, float* berrArray, int* infoArray);
+
+/* Registers caller-created HIP events (hipEvent_t, passed as void*) that the
+ * calls made of several kernel launches record on the handle's stream: event
+ * 2i just before and 2i+1 just after their i-th launch (no counterpart in the
+ * reference: a profiling aid, since stream events placed around a call cannot
+ * separate its kernels).  Honoured by aclblasSgesvBatchedMixed(Ex): launch 0
+ * factors every system, 1 solves and refines, 2 and 3 re-factor and re-solve
+ * the systems that fell back to FP32 updates (BF16 modes only).  Slots past
+ * `count` or NULL entries are not recorded; count = 0 clears the list.  The
+ * events must outlive their use. */
+aclblasStatus_t aclblasSetLaunchEvents(aclblasHandle_t handle, void* const events[], int count);
 
 /* ---- launch tuning (no reference counterpart: the reference picks its tiles
  * with fixed host rules, e.g. blas/gemm_grouped_batched/arch35/

@@ -65,6 +65,9 @@ _SIGNATURES = {
     "aclblasSgetrsBatchedAdjoint": [_P, _I, _I, _P, _I, _P, _P, _I, _P, _I, _P, _I, _I],
     "aclblasSgemmGroupedBatched": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "aclblasSgesvBatchedMixed": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, _I, _P, _P, _P],
+    "aclblasSetLaunchEvents": [_P, _P, _I],
+    "aclblasStrsm": [_P, _I, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I],
+    "aclblasSgesvBatchedMixedEx": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, _I, _I, _P, _P, _P, _P],
     "aclblasGemmExScaled": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _P, _P, _I, _I, _P, _P, _P, _I, _I],
     "aclblasQuantizeFp8": [_P, ctypes.c_longlong, _P, _I, _P, _P],
     "aclblasSetTuningOverride": [_P, _I, _I, _I],
@@ -239,6 +242,36 @@ def sgesv_batched_mixed(h, n, nrhs, A, lda, Bp, ldb, X, ldx, tol, max_iter, iter
             _host_array(ctypes.c_void_p, X), _host_array(ctypes.c_int, ldx)]
     ptrs = [ctypes.cast(a, ctypes.c_void_p) if a is not None else None for a in arrs]
     return load().aclblasSgesvBatchedMixed(h, count, *ptrs, float(tol), int(max_iter), iters, berr, info)
+
+
+SIDE = {"LEFT": 141, "RIGHT": 142}
+FILL = {"UPPER": 121, "LOWER": 122}
+DIAG = {"NON_UNIT": 131, "UNIT": 132}
+
+
+def strsm(h, side, uplo, trans, diag, m, n, alpha, A, lda, Bp, ldb):
+    """alpha: Python float (or None for a NULL pointer); A, Bp: device addresses."""
+    return load().aclblasStrsm(h, side, uplo, trans, diag, m, n, _fp(alpha), A, lda, Bp, ldb)
+
+
+def set_launch_events(h, events, count):
+    """events: a ctypes array of hipEvent_t handles (or None with count 0)."""
+    return load().aclblasSetLaunchEvents(h, events, count)
+
+
+MIXED_UPDATE = {"FP32": 0, "BF16": 1, "BF16X2": 2, "BF16X3": 3}
+
+
+def sgesv_batched_mixed_ex(h, n, nrhs, A, lda, Bp, ldb, X, ldx, tol, max_iter, update, iters=None, berr=None,
+                           info=None, fallback=None):
+    """As sgesv_batched_mixed, with the trailing-update precision (MIXED_UPDATE value) and the fallback array."""
+    count = len(n) if n is not None else 0
+    arrs = [_host_array(ctypes.c_int, n), _host_array(ctypes.c_int, nrhs), _host_array(ctypes.c_void_p, A),
+            _host_array(ctypes.c_int, lda), _host_array(ctypes.c_void_p, Bp), _host_array(ctypes.c_int, ldb),
+            _host_array(ctypes.c_void_p, X), _host_array(ctypes.c_int, ldx)]
+    ptrs = [ctypes.cast(a, ctypes.c_void_p) if a is not None else None for a in arrs]
+    return load().aclblasSgesvBatchedMixedEx(h, count, *ptrs, float(tol), int(max_iter), int(update), iters, berr,
+                                             info, fallback)
 
 
 def gemm_ex_scaled(h, ta, tb, m, n, k, alpha, A, atype, lda, scale_a, B, btype, ldb, scale_b, beta, C, ctype, ldc):

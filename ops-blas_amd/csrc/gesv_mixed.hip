@@ -1,6 +1,7 @@
-// aclblasSgesvBatchedMixed: batched dense solves of mixed orders (n <= 256,
-// nrhs <= 16) with an FP32-accurate LU whose trailing updates run on BF16
-// matrix cores, and FP64-residual iterative refinement.
+// aclblasSgesvBatchedMixed / aclblasSgesvBatchedMixedEx: batched dense solves
+// of mixed orders (n <= 256, nrhs <= 16) with a blocked LU whose trailing
+// updates run on matrix cores in a chosen precision, and FP64-residual
+// iterative refinement.
 //
 // The reference library has no such entry.  Its building blocks follow the
 // reference's getrf/getrs semantics (partial pivoting, LAPACK-style pivot
@@ -9,31 +10,56 @@
 // blocked and mixed-precision, so its results are compared with an FP64
 // solver through the backward error, not bit by bit.
 //
-// Factorisation (GesvFactorKernel, persistent, one 256-thread workgroup per
-// system, systems in order of decreasing n).  The system is copied to an FP32
-// workspace and factored in panels of 32 columns:
-//   1. wave 0 holds the panel (up to 256 rows x 32 columns, four rows per
-//      lane) in registers and factors it with partial pivoting: rows keep their
-//      place and carry their position, the pivot search is a DPP reduction of
-//      |a| keys (lowest position on exact ties), the pivot row reaches the other
-//      lanes through LDS, multipliers are a * (1/p);
-//   2. the panel is written back in pivoted row order; the panel's row map
-//      (new position -> old row) is kept for the solves, and -L21 is split into
-//      three BF16 planes (hi + mid + lo = the FP32 value) in LDS;
-//   3. all four waves sweep the trailing column tiles (16 columns each): the
-//      tile's rows are gathered through the row map straight into MFMA
-//      accumulators, the top 32 rows are solved with L11 (U12), U12 is split
-//      into BF16 planes, and A22 += (-L21) U12 runs as six
-//      v_mfma_f32_16x16x32_bf16 per 16 x 16 block (the three cross terms below
-//      2^-24 relative are dropped), then the tile is stored back.
+// Trailing-update precisions (aclblasMixedUpdate_t):
+//   FP32    A22 += (-L21) U12 on the FP32 matrix instruction
+//           (v_mfma_f32_16x16x4_f32, eight per 16 x 16 x 32 block);
+//   BF16    both operands rounded to BF16, one v_mfma_f32_16x16x32_bf16;
+//   BF16X2  both operands split into two BF16 planes (hi + lo, about 16
+//           significant bits), three products (lo*hi, hi*lo, hi*hi);
+//   BF16X3  three planes (the FP32 value exactly), six products (the terms
+//           below 2^-24 relative dropped).
+// Panels, pivoting, the 32 x 32 diagonal-block inverses, U12 and every
+// accumulation are FP32 in all modes.
+//
+// Factorisation (GesvFactorKernel<P>, persistent, one 256-thread workgroup
+// per system, systems in order of decreasing n).  The system is copied to an
+// FP32 working copy and factored in panels of 32 columns:
+//   1. the four waves hold the panel's rows (one row per lane) in registers
+//      and factor it with partial pivoting: rows keep their place and carry
+//      their position; per column each wave reduces its |a| keys with DPP
+//      (lowest position on exact ties), publishes its best row in LDS, and
+//      after one barrier every lane picks the same winner; multipliers are
+//      a * (1/p);
+//   2. the panel is written back in pivoted row order with its row map (new
+//      position -> old row), L11 kept in LDS and -L21 in the update's operand
+//      format;
+//   3. inv(L11) (32 x 32, identity-padded for a short last panel) by column
+//      substitution, kept for U12 and for the solves;
+//   4. the four waves sweep the trailing 16-column tiles: a tile's rows are
+//      gathered through the row map into MFMA accumulators (one row per lane
+//      of a 16-lane group, four columns per group, so a group reads 64
+//      contiguous bytes), U12 = inv(L11) A12 runs on the FP32 matrix
+//      instruction, U12 is stored in the update's operand format, and
+//      A22 += (-L21) U12 in the chosen precision with the accumulators in the
+//      transposed orientation (U12^T times (-L21)^T); the tile goes back to
+//      the working copy in position order.
 // Columns left of a panel are not permuted by later panels; the solves apply
 // each panel's row map just before that panel's columns, which is equivalent.
 //
 // Solve and refinement (GesvRefineKernel, persistent, one workgroup per
-// system): x0 from the FP32 factors, then until every right-hand side meets
-// |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol (residual in FP64 from
-// the original FP32 A and b, x kept in FP64) or maxIter corrections have been
-// applied: d = LU \ r in FP32, x += d.
+// system): x0 from the factors, then until every right-hand side meets
+// |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol, or the refinement stops
+// contracting, or maxIter corrections have been applied: r = b - A x in FP64
+// (A read in FP32, x kept in FP64 in LDS), d = LU \ r in FP32, x += d; a
+// correction that increases the backward error is undone.  The triangular solves run blockwise on the FP32
+// matrix instruction: each 32-row block is multiplied by its stored diagonal
+// inverse, the rows below (or above) are updated by the block's L21 (U12).
+//
+// Fallback: with a BF16 mode, a system whose refinement does not reach tol
+// (the backward error fails to halve in a correction, maxIter is exhausted,
+// a zero pivot or a NaN appears) is marked in the workspace; two more
+// launches re-factor the marked systems with FP32 updates and solve them
+// again (every other system is skipped by a flag read).
 #include <algorithm>
 #include <vector>
 
@@ -54,8 +80,8 @@ constexpr int kThreads = 256;            // four waves
 constexpr int kWaves = kThreads / kWave;
 constexpr int kTileN = 16;               // sweep tile width
 constexpr int kMaxTiles = kMaxN / 16;    // 16-row accumulator tiles per column tile
-constexpr int kLStride = kNb;            // bf16 elements per L21 plane row (64 bytes, 16-byte aligned)
-constexpr int kUStride = kNb;            // bf16 elements per U12 plane row
+constexpr int kLd = kNb + 4;             // row stride (floats) of 32-wide FP32 LDS blocks: 16-byte rows
+constexpr int kInvFloats = kNb * kNb;    // per panel: inv(L11), 32 x 32 column-major
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -64,9 +90,10 @@ struct MixedSystem {
     const float* A;
     const float* B;
     float* X;
-    float* LU;       // workspace: n x n, leading dimension n
+    float* LU;       // workspace: n x n working copy, leading dimension n
+    float* dinv;     // workspace: per panel inv(L11)
     int* rowmap;     // workspace: per-panel row maps, panel p at p*n - 16*p*(p-1)
-    int* status;     // workspace: 0, or k+1 for a zero pivot at column k
+    int* state;      // workspace: [0] zero-pivot column + 1 or 0; [1] 1 = solve again with FP32 updates
     int n, nrhs, lda, ldb, ldx, index;
 };
 
@@ -83,25 +110,44 @@ __device__ __forceinline__ uint16_t Bf16Rne(float f)
 }
 __device__ __forceinline__ float Bf16ToF(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 
-// x = hi + mid + lo exactly for finite x of normal range.
-__device__ __forceinline__ void Split3(float x, uint16_t& hi, uint16_t& mid, uint16_t& lo)
+// x = sum of the P planes: each plane is the BF16 rounding of what the planes
+// before it leave (three planes hold every finite FP32 value of normal range
+// exactly).
+template <int P>
+__device__ __forceinline__ void SplitPlanes(float x, uint16_t (&h)[P])
 {
-    hi = Bf16Rne(x);
-    const float r1 = x - Bf16ToF(hi);
-    mid = Bf16Rne(r1);
-    const float r2 = r1 - Bf16ToF(mid);
-    lo = Bf16Rne(r2);
+    float r = x;
+    dev::StaticFor<0, P>([&](auto pc) {
+        constexpr int p = decltype(pc)::value;
+        h[p] = Bf16Rne(r);
+        if constexpr (p + 1 < P) {
+            r = r - Bf16ToF(h[p]);
+        }
+    });
 }
 
+// -L21 and U12 of the current panel in the operand format of the update.
+template <int P>
+struct UpdateOperands {  // P BF16 planes each
+    alignas(16) uint16_t l[P][kMaxN - kNb][kNb];          // -L21 [plane][row][k]
+    alignas(16) uint16_t u[kWaves][P][kTileN][kNb];       // U12^T per wave [plane][col][k]
+};
+template <>
+struct UpdateOperands<0> {  // FP32
+    alignas(16) float l[kMaxN - kNb][kLd];                // -L21 [row][k]
+    alignas(16) float u[kWaves][kTileN][kLd];             // U12^T per wave [col][k]
+};
+
+template <int P>
 struct FactorShared {
-    alignas(16) float cand[2][kWaves][kNb];                  // each wave's best row, by column parity
+    alignas(16) float cand[2][kWaves][kNb];               // each wave's best row, by column parity
     uint32_t candKey[2][kWaves];
     int candPos[2][kWaves];
-    alignas(16) uint16_t lplanes[3][kMaxN - kNb][kLStride];  // -L21 split into BF16 planes
-    alignas(16) uint16_t uplanes[kWaves][3][kTileN][kUStride];
-    int rowmap[kMaxN];                                       // panel row map
-    float l11[kNb][kNb + 1];                                 // panel rows 0..31 (FP32)
-    float ut[kWaves][kNb][kTileN + 1];                       // per-wave TRSM tile
+    int rowmap[kMaxN];                                    // panel row map
+    alignas(16) float l11[kNb][kLd];                      // panel rows 0..31: L11 below, U11 on/above the diagonal
+    alignas(16) float linv[kNb][kLd];                     // inv(L11), row-major
+    alignas(16) float ut[kWaves][kNb][kTileN + 1];        // per-wave top rows of the sweep tile
+    UpdateOperands<P> op;
 };
 
 __device__ __forceinline__ void WaveSync()
@@ -117,7 +163,8 @@ __device__ __forceinline__ void WaveSync()
 // position in LDS (double-buffered by column parity), one barrier, and every
 // lane picks the same winner among the four.  Returns the zero-pivot column
 // + 1, or 0 (uniform over the workgroup).
-__device__ __forceinline__ int FactorPanel(const MixedSystem& s, int k0, FactorShared& sh)
+template <int P>
+__device__ __forceinline__ int FactorPanel(const MixedSystem& s, int k0, FactorShared<P>& sh)
 {
     const int n = s.n;
     const int lane = dev::LaneId();
@@ -212,7 +259,7 @@ __device__ __forceinline__ int FactorPanel(const MixedSystem& s, int k0, FactorS
         return zeroCol;
     }
 
-    // write back in pivoted order; row map; L11 (FP32) and -L21 planes (BF16)
+    // write back in pivoted order; row map; L11/U11 (FP32) and -L21 in the update's format
     int* gmap = s.rowmap + RowMapOffset(n, k0 / kNb);
     if (valid) {
         const int p = pos;
@@ -225,18 +272,26 @@ __device__ __forceinline__ int FactorPanel(const MixedSystem& s, int k0, FactorS
             }
         });
         if (p < kNb) {
-            dev::StaticFor<0, kNb>([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
-                sh.l11[p][j] = a[j];
+            dev::StaticFor<0, kNb / 4>([&](auto cc) {
+                constexpr int c = decltype(cc)::value;
+                *reinterpret_cast<float4*>(&sh.l11[p][4 * c]) =
+                    make_float4(a[4 * c], a[4 * c + 1], a[4 * c + 2], a[4 * c + 3]);
+            });
+        } else if constexpr (P == 0) {
+            dev::StaticFor<0, kNb / 4>([&](auto cc) {
+                constexpr int c = decltype(cc)::value;
+                *reinterpret_cast<float4*>(&sh.op.l[p - kNb][4 * c]) =
+                    make_float4(-a[4 * c], -a[4 * c + 1], -a[4 * c + 2], -a[4 * c + 3]);
             });
         } else {
             dev::StaticFor<0, kNb>([&](auto jc) {
                 constexpr int j = decltype(jc)::value;
-                uint16_t h, m, l;
-                Split3(-a[j], h, m, l);
-                sh.lplanes[0][p - kNb][j] = h;
-                sh.lplanes[1][p - kNb][j] = m;
-                sh.lplanes[2][p - kNb][j] = l;
+                uint16_t h[P];
+                SplitPlanes<P>(-a[j], h);
+                dev::StaticFor<0, P>([&](auto pc) {
+                    constexpr int q = decltype(pc)::value;
+                    sh.op.l[q][p - kNb][j] = h[q];
+                });
             });
         }
     }
@@ -244,16 +299,51 @@ __device__ __forceinline__ int FactorPanel(const MixedSystem& s, int k0, FactorS
     return 0;
 }
 
-// One 16-column tile of the trailing matrix: gather, TRSM, BF16x3 update, store.
-__device__ __forceinline__ void SweepTile(const MixedSystem& s, int k0, int c0, FactorShared& sh)
+// inv(L11) by wave 0, one column per lane, by column substitution; rows and
+// columns past the panel's width w are the identity's.  It goes to LDS (for
+// U12) and to the workspace (for the solves' diagonal blocks).
+template <int P>
+__device__ __forceinline__ void PanelInverses(const MixedSystem& s, int k0, int p, FactorShared<P>& sh)
+{
+    const int lane = dev::LaneId();
+    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    const int w = min(kNb, s.n - k0);
+    float* g = s.dinv + static_cast<size_t>(p) * kInvFloats;
+    if (wave == 0 && lane < kNb) {
+        const int j = lane;
+        float x[kNb];
+        dev::StaticFor<0, kNb>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            float v = q == j ? 1.0f : 0.0f;
+            if (q < w) {
+                dev::StaticFor<0, q>([&](auto ic) {
+                    constexpr int i = decltype(ic)::value;
+                    v = fmaf(-sh.l11[q][i], x[i], v);
+                });
+            }
+            x[q] = v;
+        });
+        dev::StaticFor<0, kNb>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            sh.linv[q][j] = x[q];
+            g[q + kNb * j] = x[q];
+        });
+    }
+}
+
+// One 16-column tile of the trailing matrix: gather, U12 = inv(L11) A12,
+// update, store.  Accumulator t, element j of a lane holds row 16t + (lane & 15)
+// (panel position) and column c0 + 4 (lane >> 4) + j.
+template <int P>
+__device__ __forceinline__ void SweepTile(const MixedSystem& s, int k0, int c0, FactorShared<P>& sh)
 {
     const int n = s.n;
     const int lane = dev::LaneId();
     const int wave = static_cast<int>(threadIdx.x) / kWave;
     const int rows = n - k0;
-    const int col = c0 + (lane & 15);
-    const bool colOk = col < n;
-    const int rq = (lane >> 4) * 4;
+    const int r16 = lane & 15;
+    const int grp = lane >> 4;
+    const int cq = grp * 4;
     float* LU = s.LU;
     const int tiles = (rows + 15) / 16;
 
@@ -261,104 +351,143 @@ __device__ __forceinline__ void SweepTile(const MixedSystem& s, int k0, int c0, 
     dev::StaticFor<0, kMaxTiles>([&](auto tc) {
         constexpr int t = decltype(tc)::value;
         if (t < tiles) {
-            dev::StaticFor<0, 4>([&](auto ic) {
-                constexpr int i = decltype(ic)::value;
-                const int row = 16 * t + rq + i;
-                acc[t][i] = (row < rows && colOk) ? LU[(k0 + sh.rowmap[row]) + static_cast<size_t>(col) * n] : 0.0f;
+            const int row = 16 * t + r16;
+            const bool ok = row < rows;
+            const size_t src = static_cast<size_t>(k0 + (ok ? sh.rowmap[row] : 0));
+            dev::StaticFor<0, 4>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                const int col = c0 + cq + j;
+                acc[t][j] = (ok && col < n) ? LU[src + static_cast<size_t>(col) * n] : 0.0f;
             });
         }
     });
 
-    // U12 = L11^-1 (top 32 rows): through the per-wave LDS tile
+    // U12 = inv(L11) A12 (top 32 rows) on the FP32 matrix instruction, as
+    // U12^T = A12^T inv(L11)^T: A12 through the per-wave LDS tile
     float (*ut)[kTileN + 1] = sh.ut[wave];
     dev::StaticFor<0, 2>([&](auto tc) {
         constexpr int t = decltype(tc)::value;
-        dev::StaticFor<0, 4>([&](auto ic) {
-            constexpr int i = decltype(ic)::value;
-            ut[16 * t + rq + i][lane & 15] = acc[t][i];
+        dev::StaticFor<0, 4>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            ut[16 * t + r16][cq + j] = acc[t][j];
         });
     });
     WaveSync();
-    if (lane < kTileN) {
-        float x[kNb];
-        dev::StaticFor<0, kNb>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            float v = ut[q][lane];
-            dev::StaticFor<0, q>([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
-                v = fmaf(-sh.l11[q][j], x[j], v);
-            });
-            x[q] = v;
-            ut[q][lane] = v;
+    dev::StaticFor<0, 2>([&](auto bc) {
+        constexpr int b = decltype(bc)::value;
+        f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+        dev::StaticFor<0, kNb / 4>([&](auto sc) {
+            constexpr int st = decltype(sc)::value;
+            v = __builtin_amdgcn_mfma_f32_16x16x4f32(ut[4 * st + grp][r16], sh.linv[16 * b + r16][4 * st + grp], v,
+                                                     0, 0, 0);
         });
-        dev::StaticFor<0, kNb>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            uint16_t h, m, l;
-            Split3(x[q], h, m, l);
-            sh.uplanes[wave][0][lane][q] = h;
-            sh.uplanes[wave][1][lane][q] = m;
-            sh.uplanes[wave][2][lane][q] = l;
-        });
-    }
-    WaveSync();
-    dev::StaticFor<0, 2>([&](auto tc) {
-        constexpr int t = decltype(tc)::value;
-        dev::StaticFor<0, 4>([&](auto ic) {
-            constexpr int i = decltype(ic)::value;
-            acc[t][i] = ut[16 * t + rq + i][lane & 15];
-        });
+        acc[b] = v;
     });
 
-    // A22 += (-L21) U12 with split BF16 operands (small terms first)
-    const int kq = (lane >> 4) * 8;
-    bf16x8 ub[3];
-    dev::StaticFor<0, 3>([&](auto pc) {
-        constexpr int p = decltype(pc)::value;
-        ub[p] = *reinterpret_cast<const bf16x8*>(&sh.uplanes[wave][p][lane & 15][kq]);
+    // U12 in the update's operand format (U12^T: [col][k])
+    dev::StaticFor<0, 2>([&](auto bc) {
+        constexpr int b = decltype(bc)::value;
+        dev::StaticFor<0, 4>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            const int col = cq + j;
+            const int k = 16 * b + r16;
+            if constexpr (P == 0) {
+                sh.op.u[wave][col][k] = acc[b][j];
+            } else {
+                uint16_t h[P];
+                SplitPlanes<P>(acc[b][j], h);
+                dev::StaticFor<0, P>([&](auto pc) {
+                    constexpr int q = decltype(pc)::value;
+                    sh.op.u[wave][q][col][k] = h[q];
+                });
+            }
+        });
     });
-    dev::StaticFor<2, kMaxTiles>([&](auto tc) {
-        constexpr int t = decltype(tc)::value;
-        if (t < tiles) {
-            const int lr = 16 * t - kNb + (lane & 15);
-            bf16x8 la[3];
-            dev::StaticFor<0, 3>([&](auto pc) {
-                constexpr int p = decltype(pc)::value;
-                la[p] = *reinterpret_cast<const bf16x8*>(&sh.lplanes[p][lr][kq]);
-            });
-            f32x4 v = acc[t];
-            v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(la[2], ub[0], v, 0, 0, 0);
-            v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(la[0], ub[2], v, 0, 0, 0);
-            v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(la[1], ub[1], v, 0, 0, 0);
-            v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(la[1], ub[0], v, 0, 0, 0);
-            v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(la[0], ub[1], v, 0, 0, 0);
-            v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(la[0], ub[0], v, 0, 0, 0);
-            acc[t] = v;
-        }
-    });
+    WaveSync();
+
+    // A22 += (-L21) U12, as acc^T += U12^T (-L21)^T, small terms first
+    if constexpr (P == 0) {
+        float ua[kNb / 4];
+        dev::StaticFor<0, kNb / 4>([&](auto sc) {
+            constexpr int st = decltype(sc)::value;
+            ua[st] = sh.op.u[wave][r16][4 * st + grp];
+        });
+        dev::StaticFor<2, kMaxTiles>([&](auto tc) {
+            constexpr int t = decltype(tc)::value;
+            if (t < tiles) {
+                const int lr = 16 * t - kNb + r16;
+                f32x4 v = acc[t];
+                dev::StaticFor<0, kNb / 4>([&](auto sc) {
+                    constexpr int st = decltype(sc)::value;
+                    v = __builtin_amdgcn_mfma_f32_16x16x4f32(ua[st], sh.op.l[lr][4 * st + grp], v, 0, 0, 0);
+                });
+                acc[t] = v;
+            }
+        });
+    } else {
+        const int kq = grp * 8;
+        bf16x8 ub[P];
+        dev::StaticFor<0, P>([&](auto pc) {
+            constexpr int q = decltype(pc)::value;
+            ub[q] = *reinterpret_cast<const bf16x8*>(&sh.op.u[wave][q][r16][kq]);
+        });
+        dev::StaticFor<2, kMaxTiles>([&](auto tc) {
+            constexpr int t = decltype(tc)::value;
+            if (t < tiles) {
+                const int lr = 16 * t - kNb + r16;
+                bf16x8 la[P];
+                dev::StaticFor<0, P>([&](auto pc) {
+                    constexpr int q = decltype(pc)::value;
+                    la[q] = *reinterpret_cast<const bf16x8*>(&sh.op.l[q][lr][kq]);
+                });
+                f32x4 v = acc[t];
+                if constexpr (P == 3) {
+                    v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ub[0], la[2], v, 0, 0, 0);
+                    v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ub[2], la[0], v, 0, 0, 0);
+                    v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ub[1], la[1], v, 0, 0, 0);
+                }
+                if constexpr (P >= 2) {
+                    v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ub[0], la[1], v, 0, 0, 0);
+                    v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ub[1], la[0], v, 0, 0, 0);
+                }
+                v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ub[0], la[0], v, 0, 0, 0);
+                acc[t] = v;
+            }
+        });
+    }
 
     dev::StaticFor<0, kMaxTiles>([&](auto tc) {
         constexpr int t = decltype(tc)::value;
         if (t < tiles) {
-            dev::StaticFor<0, 4>([&](auto ic) {
-                constexpr int i = decltype(ic)::value;
-                const int row = 16 * t + rq + i;
-                if (row < rows && colOk) {
-                    LU[(k0 + row) + static_cast<size_t>(col) * n] = acc[t][i];
-                }
-            });
+            const int row = 16 * t + r16;
+            if (row < rows) {
+                dev::StaticFor<0, 4>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    const int col = c0 + cq + j;
+                    if (col < n) {
+                        LU[(k0 + row) + static_cast<size_t>(col) * n] = acc[t][j];
+                    }
+                });
+            }
         }
     });
     WaveSync();
 }
 
-__global__ __launch_bounds__(kThreads, 2) void GesvFactorKernel(const MixedSystem* __restrict__ systems, int count)
+// onlyMarked: factor only the systems the refinement marked for FP32 updates.
+template <int P>
+__global__ __launch_bounds__(kThreads, 2) void GesvFactorKernel(const MixedSystem* __restrict__ systems, int count,
+                                                            int onlyMarked)
 {
-    __shared__ __attribute__((aligned(16))) FactorShared sh;
+    __shared__ __attribute__((aligned(16))) FactorShared<P> sh;
     __shared__ int failed;
     const int t = threadIdx.x;
     const int wave = t / kWave;
     for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
         const MixedSystem s = systems[idx];
+        if (onlyMarked != 0 && s.state[1] == 0) {
+            continue;
+        }
         const int n = s.n;
         // working copy of A
         if (t < n) {
@@ -380,22 +509,24 @@ __global__ __launch_bounds__(kThreads, 2) void GesvFactorKernel(const MixedSyste
             failed = 0;
         }
         __syncthreads();
-        for (int k0 = 0; k0 < n; k0 += kNb) {
-            const int z = FactorPanel(s, k0, sh);
+        for (int k0 = 0, p = 0; k0 < n; k0 += kNb, ++p) {
+            const int z = FactorPanel<P>(s, k0, sh);
             if (z != 0) {
                 if (t == 0) {
                     failed = z;
                 }
                 break;
             }
+            PanelInverses<P>(s, k0, p, sh);
+            __syncthreads();
             const int first = k0 + kNb;  // first trailing column (panels before the last are full)
             for (int c0 = first + wave * kTileN; c0 < n; c0 += kWaves * kTileN) {
-                SweepTile(s, k0, c0, sh);
+                SweepTile<P>(s, k0, c0, sh);
             }
             __syncthreads();
         }
         if (t == 0) {
-            *s.status = failed;
+            s.state[0] = failed;
         }
         __syncthreads();
     }
@@ -405,129 +536,114 @@ __global__ __launch_bounds__(kThreads, 2) void GesvFactorKernel(const MixedSyste
 // Solve and refinement
 // ---------------------------------------------------------------------------
 struct RefineShared {
-    float bs[kMaxN][kMaxRhs];
-    double x[kMaxN][kMaxRhs];
-    double red[3][kWaves][kMaxRhs];  // per-wave max |r|, |x|, |b|
-    double anorm[kWaves];            // per-wave max row sum of |A|
+    alignas(16) float bs[kMaxN][kMaxRhs];     // right-hand sides / solutions (FP32); rows >= n stay 0
+    alignas(16) double x[kMaxN][kMaxRhs];     // current solution (FP64); rows >= n and columns >= nrhs stay 0
+    double red[3][kWaves][kMaxRhs];           // per-wave max |r|, |x|, |b|
+    double anorm[kWaves];                     // per-wave max row sum of |A|
     double berr[kMaxRhs];
 };
 
-__device__ __forceinline__ double WaveMaxD(double v)
-{
-    for (int off = 32; off >= 1; off >>= 1) {
-        const double o = __shfl_xor(v, off, kWave);
-        v = o > v ? o : v;
-    }
-    return v;
-}
+__device__ __forceinline__ double MaxD(double a, double b) { return b > a ? b : a; }
 
-// bs <- LU^-1 bs (forward with per-panel row maps, then backward).
+// bs <- LU^-1 bs: per panel the row map, inv(L11) on the diagonal block and
+// the block's L21 below it; then from the bottom each diagonal block of U by
+// substitution and its U column block above it.  FP32 matrix instruction: a
+// lane holds rows 4 (lane >> 4) + j of a 16-row block and right-hand side
+// lane & 15.
 __device__ __forceinline__ void FactoredSolve(const MixedSystem& s, RefineShared& sh)
 {
     const int n = s.n;
-    const int nrhs = s.nrhs;
     const int t = threadIdx.x;
     const int lane = dev::LaneId();
     const int wave = t / kWave;
+    const int r16 = lane & 15;
+    const int grp = lane >> 4;
     const float* LU = s.LU;
-    for (int k0 = 0, p = 0; k0 < n; k0 += kNb, ++p) {
+    const int blocks = (n + kNb - 1) / kNb;
+    for (int p = 0; p < blocks; ++p) {
+        const int k0 = p * kNb;
         const int rows = n - k0;
-        const int w = min(kNb, rows);
-        const int* map = s.rowmap + RowMapOffset(n, p);
         {
-            // gather through the row map in registers, then write back in place
-            float g[kMaxRhs];
+            // row interchanges of this panel: gather through the map in registers, write back
+            const int* map = s.rowmap + RowMapOffset(n, p);
             const int src = t < rows ? map[t] : 0;
-            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                g[c] = (t < rows && c < nrhs) ? sh.bs[k0 + src][c] : 0.0f;
-            });
+            float g[kMaxRhs];
+            if (t < rows) {
+                dev::StaticFor<0, kMaxRhs / 4>([&](auto cc) {
+                    constexpr int c = decltype(cc)::value;
+                    const float4 v = *reinterpret_cast<const float4*>(&sh.bs[k0 + src][4 * c]);
+                    g[4 * c] = v.x;
+                    g[4 * c + 1] = v.y;
+                    g[4 * c + 2] = v.z;
+                    g[4 * c + 3] = v.w;
+                });
+            }
             __syncthreads();
-            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                if (t < rows && c < nrhs) {
-                    sh.bs[k0 + t][c] = g[c];
-                }
+            if (t < rows) {
+                dev::StaticFor<0, kMaxRhs / 4>([&](auto cc) {
+                    constexpr int c = decltype(cc)::value;
+                    *reinterpret_cast<float4*>(&sh.bs[k0 + t][4 * c]) =
+                        make_float4(g[4 * c], g[4 * c + 1], g[4 * c + 2], g[4 * c + 3]);
+                });
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {
+            // diagonal block: y = inv(L11) b (rows past n are zero in b, identity in the inverse)
+            const float* Li = s.dinv + static_cast<size_t>(p) * kInvFloats;
+            f32x4 y[2];
+            dev::StaticFor<0, 2>([&](auto bc) {
+                constexpr int b = decltype(bc)::value;
+                f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+                dev::StaticFor<0, kNb / 4>([&](auto sc) {
+                    constexpr int st = decltype(sc)::value;
+                    const int k = 4 * st + grp;
+                    v = __builtin_amdgcn_mfma_f32_16x16x4f32(Li[(16 * b + r16) + kNb * k], sh.bs[k0 + k][r16], v, 0,
+                                                             0, 0);
+                });
+                y[b] = v;
+            });
+            dev::StaticFor<0, 2>([&](auto bc) {
+                constexpr int b = decltype(bc)::value;
+                dev::StaticFor<0, 4>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    sh.bs[k0 + 16 * b + 4 * grp + j][r16] = y[b][j];
+                });
             });
         }
         __syncthreads();
-        {
-            // diagonal block: lane q holds row q; wave v solves right-hand sides c = v, v+4, ...
-            const bool valid = lane < w;
-            float l[kNb];
-            float x[kMaxRhs / kWaves];
-            dev::StaticFor<0, kNb>([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
-                l[j] = (valid && j < lane) ? LU[(k0 + lane) + static_cast<size_t>(k0 + j) * n] : 0.0f;
-            });
-            dev::StaticFor<0, kMaxRhs / kWaves>([&](auto cc) {
-                constexpr int ci = decltype(cc)::value;
-                const int c = ci * kWaves + wave;
-                x[ci] = (valid && c < nrhs) ? sh.bs[k0 + lane][c] : 0.0f;
-            });
-            dev::StaticFor<0, kNb>([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
-                if (j < w) {
-                    dev::StaticFor<0, kMaxRhs / kWaves>([&](auto cc) {
-                        constexpr int ci = decltype(cc)::value;
-                        if (ci * kWaves + wave < nrhs) {
-                            const float xj = dev::ReadLane(x[ci], j);
-                            if (lane > j) {
-                                x[ci] = fmaf(-l[j], xj, x[ci]);
-                            }
-                        }
-                    });
-                }
-            });
-            if (valid) {
-                dev::StaticFor<0, kMaxRhs / kWaves>([&](auto cc) {
-                    constexpr int ci = decltype(cc)::value;
-                    const int c = ci * kWaves + wave;
-                    if (c < nrhs) {
-                        sh.bs[k0 + lane][c] = x[ci];
+        // rows below the block: b -= L21 y
+        const int below = rows - kNb;
+        if (below > 0) {
+            const int nb16 = (below + 15) / 16;
+            for (int blk = wave; blk < nb16; blk += kWaves) {
+                const int r0 = k0 + kNb + 16 * blk;
+                const int row = r0 + r16;
+                f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+                dev::StaticFor<0, kNb / 4>([&](auto sc) {
+                    constexpr int st = decltype(sc)::value;
+                    const int k = 4 * st + grp;
+                    const float a = row < n ? LU[row + static_cast<size_t>(k0 + k) * n] : 0.0f;
+                    v = __builtin_amdgcn_mfma_f32_16x16x4f32(a, sh.bs[k0 + k][r16], v, 0, 0, 0);
+                });
+                dev::StaticFor<0, 4>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    const int rr = r0 + 4 * grp + j;
+                    if (rr < n) {
+                        sh.bs[rr][r16] -= v[j];
                     }
                 });
             }
         }
         __syncthreads();
-        const int i = k0 + w + t;
-        if (i < n) {
-            float acc[kMaxRhs];
-            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                acc[c] = c < nrhs ? sh.bs[i][c] : 0.0f;
-            });
-            float lv[kNb];
-            dev::StaticFor<0, kNb>([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
-                lv[j] = j < w ? LU[i + static_cast<size_t>(k0 + j) * n] : 0.0f;
-            });
-            dev::StaticFor<0, kNb>([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
-                if (j < w) {
-                    dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                        constexpr int c = decltype(cc)::value;
-                        if (c < nrhs) {
-                            acc[c] = fmaf(-lv[j], sh.bs[k0 + j][c], acc[c]);
-                        }
-                    });
-                }
-            });
-            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                if (c < nrhs) {
-                    sh.bs[i][c] = acc[c];
-                }
-            });
-        }
-        __syncthreads();
     }
-    // backward: 32-row blocks from the bottom
-    const int blocks = (n + kNb - 1) / kNb;
     for (int bi = blocks - 1; bi >= 0; --bi) {
         const int r0 = bi * kNb;
         const int w = min(kNb, n - r0);
         {
+            // diagonal block by substitution (U_bb can be ill conditioned: no
+            // inverse here): lane q holds row q of U_bb, wave v the right-hand
+            // sides v, v + 4, ...
             const bool valid = lane < w;
             float u[kNb];
             float x[kMaxRhs / kWaves];
@@ -537,15 +653,14 @@ __device__ __forceinline__ void FactoredSolve(const MixedSystem& s, RefineShared
             });
             dev::StaticFor<0, kMaxRhs / kWaves>([&](auto cc) {
                 constexpr int ci = decltype(cc)::value;
-                const int c = ci * kWaves + wave;
-                x[ci] = (valid && c < nrhs) ? sh.bs[r0 + lane][c] : 0.0f;
+                x[ci] = valid ? sh.bs[r0 + lane][ci * kWaves + wave] : 0.0f;
             });
             dev::StaticFor<0, kNb>([&](auto jc) {
                 constexpr int j = kNb - 1 - decltype(jc)::value;
                 if (j < w) {
                     dev::StaticFor<0, kMaxRhs / kWaves>([&](auto cc) {
                         constexpr int ci = decltype(cc)::value;
-                        if (ci * kWaves + wave < nrhs) {
+                        if (ci * kWaves + wave < s.nrhs) {
                             if (lane == j) {
                                 x[ci] = x[ci] / u[j];
                             }
@@ -560,58 +675,145 @@ __device__ __forceinline__ void FactoredSolve(const MixedSystem& s, RefineShared
             if (valid) {
                 dev::StaticFor<0, kMaxRhs / kWaves>([&](auto cc) {
                     constexpr int ci = decltype(cc)::value;
-                    const int c = ci * kWaves + wave;
-                    if (c < nrhs) {
-                        sh.bs[r0 + lane][c] = x[ci];
-                    }
+                    sh.bs[r0 + lane][ci * kWaves + wave] = x[ci];
                 });
             }
         }
         __syncthreads();
-        if (t < r0) {
-            float acc[kMaxRhs];
-            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                acc[c] = c < nrhs ? sh.bs[t][c] : 0.0f;
+        // rows above the block: b -= U[0:r0, block] y
+        const int nb16 = r0 / 16;
+        for (int blk = wave; blk < nb16; blk += kWaves) {
+            const int row = 16 * blk + r16;
+            f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+            dev::StaticFor<0, kNb / 4>([&](auto sc) {
+                constexpr int st = decltype(sc)::value;
+                const int k = 4 * st + grp;
+                const float a = r0 + k < n ? LU[row + static_cast<size_t>(r0 + k) * n] : 0.0f;
+                v = __builtin_amdgcn_mfma_f32_16x16x4f32(a, sh.bs[r0 + k][r16], v, 0, 0, 0);
             });
-            float uv[kNb];
-            dev::StaticFor<0, kNb>([&](auto jc) {
+            dev::StaticFor<0, 4>([&](auto jc) {
                 constexpr int j = decltype(jc)::value;
-                uv[j] = j < w ? LU[t + static_cast<size_t>(r0 + j) * n] : 0.0f;
-            });
-            dev::StaticFor<0, kNb>([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
-                if (j < w) {
-                    dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                        constexpr int c = decltype(cc)::value;
-                        if (c < nrhs) {
-                            acc[c] = fmaf(-uv[j], sh.bs[r0 + j][c], acc[c]);
-                        }
-                    });
-                }
-            });
-            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                if (c < nrhs) {
-                    sh.bs[t][c] = acc[c];
-                }
+                sh.bs[16 * blk + 4 * grp + j][r16] -= v[j];
             });
         }
         __syncthreads();
     }
 }
 
+__device__ __forceinline__ double WaveMaxD(double v)
+{
+    for (int off = 32; off >= 1; off >>= 1) {
+        v = MaxD(v, __shfl_xor(v, off, kWave));
+    }
+    return v;
+}
+
+// r = b - A x in FP64 (A read in FP32, x from LDS, one row per thread: row t
+// of r stays in the caller's registers); the worst normwise backward error
+// over the right-hand sides is returned (uniform).  With withNorm, |A|_inf is
+// computed into sh.anorm first.
+__device__ __forceinline__ double Residual(const MixedSystem& s, RefineShared& sh, bool withNorm,
+                                           double (&r)[kMaxRhs])
+{
+    const int n = s.n;
+    const int nrhs = s.nrhs;
+    const int t = threadIdx.x;
+    const int lane = dev::LaneId();
+    const int wave = t / kWave;
+    double rs = 0.0;
+    dev::StaticFor<0, kMaxRhs>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        r[c] = (t < n && c < nrhs) ? static_cast<double>(s.B[t + static_cast<size_t>(c) * s.ldb]) : 0.0;
+    });
+    if (t < n) {
+        for (int j0 = 0; j0 < n; j0 += 4) {
+            float av[4];
+            dev::StaticFor<0, 4>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                av[j] = j0 + j < n ? s.A[t + static_cast<size_t>(j0 + j) * s.lda] : 0.0f;
+            });
+            dev::StaticFor<0, 4>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                const double a = static_cast<double>(av[j]);
+                rs += fabs(a);
+                // x rows past n are zero, so the last partial group needs no check
+                dev::StaticFor<0, kMaxRhs / 2>([&](auto cc) {
+                    constexpr int c = decltype(cc)::value;
+                    if (2 * c < nrhs) {
+                        const double2 xv = *reinterpret_cast<const double2*>(&sh.x[j0 + j][2 * c]);
+                        r[2 * c] = fma(-a, xv.x, r[2 * c]);
+                        r[2 * c + 1] = fma(-a, xv.y, r[2 * c + 1]);
+                    }
+                });
+            });
+        }
+    }
+    if (withNorm) {
+        const double m = WaveMaxD(rs);
+        if (lane == 0) {
+            sh.anorm[wave] = m;
+        }
+    }
+    dev::StaticFor<0, kMaxRhs>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        if (c < nrhs) {
+            const double rv = WaveMaxD(fabs(r[c]));
+            const double xv = WaveMaxD(t < n ? fabs(sh.x[t][c]) : 0.0);
+            const double bv = WaveMaxD(t < n ? fabs(static_cast<double>(s.B[t + static_cast<size_t>(c) * s.ldb]))
+                                             : 0.0);
+            if (lane == 0) {
+                sh.red[0][wave][c] = rv;
+                sh.red[1][wave][c] = xv;
+                sh.red[2][wave][c] = bv;
+            }
+        }
+    });
+    __syncthreads();
+    if (t < nrhs) {
+        double an = 0.0, rn = 0.0, xn = 0.0, bn = 0.0;
+        for (int q = 0; q < kWaves; ++q) {
+            an = MaxD(an, sh.anorm[q]);
+            rn = MaxD(rn, sh.red[0][q][t]);
+            xn = MaxD(xn, sh.red[1][q][t]);
+            bn = MaxD(bn, sh.red[2][q][t]);
+        }
+        const double den = an * xn + bn;
+        sh.berr[t] = den > 0.0 ? rn / den : (rn > 0.0 ? __builtin_inf() : 0.0);
+    }
+    __syncthreads();
+    double worst = 0.0;
+    for (int c = 0; c < nrhs; ++c) {
+        const double be = sh.berr[c];
+        worst = (be > worst || !(be == be)) ? be : worst;
+    }
+    __syncthreads();
+    return worst;
+}
+
+// pass 0: every system; with mayMark, a system that does not reach tol is
+// marked (state[1] = 1) and left for the FP32 pass.  pass 1 (onlyMarked):
+// the marked systems, final.
 __global__ __launch_bounds__(kThreads, 3) void GesvRefineKernel(const MixedSystem* __restrict__ systems, int count,
-                                                            double tol, int maxIter, int* __restrict__ iterOut,
-                                                            float* __restrict__ berrOut, int* __restrict__ infoOut)
+                                                            double tol, int maxIter, int mayMark, int onlyMarked,
+                                                            int* __restrict__ iterOut, float* __restrict__ berrOut,
+                                                            int* __restrict__ infoOut, int* __restrict__ fallbackOut)
 {
     __shared__ __attribute__((aligned(16))) RefineShared sh;
     const int t = threadIdx.x;
     for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
         const MixedSystem s = systems[idx];
+        if (onlyMarked != 0 && s.state[1] == 0) {
+            continue;
+        }
         const int n = s.n;
         const int nrhs = s.nrhs;
-        const int status = *s.status;
+        const int status = s.state[0];
+        if (status != 0 && mayMark != 0) {
+            if (t == 0) {
+                s.state[1] = 1;
+            }
+            continue;
+        }
         if (status != 0) {
             for (int c = 0; c < nrhs; ++c) {
                 for (int i = t; i < n; i += kThreads) {
@@ -622,116 +824,71 @@ __global__ __launch_bounds__(kThreads, 3) void GesvRefineKernel(const MixedSyste
                 if (iterOut != nullptr) iterOut[s.index] = 0;
                 if (berrOut != nullptr) berrOut[s.index] = __builtin_inff();
                 if (infoOut != nullptr) infoOut[s.index] = status;
+                if (fallbackOut != nullptr) fallbackOut[s.index] = onlyMarked;
+                s.state[1] = 0;
             }
             continue;
         }
-        // x0 = LU \ b
-        if (t < n) {
-            for (int c = 0; c < nrhs; ++c) {
-                sh.bs[t][c] = s.B[t + static_cast<size_t>(c) * s.ldb];
-            }
-        }
+        // x0 = LU \ b (rows >= n and columns >= nrhs of bs and x are zero)
+        dev::StaticFor<0, kMaxRhs>([&](auto cc) {
+            constexpr int c = decltype(cc)::value;
+            sh.bs[t][c] = (t < n && c < nrhs) ? s.B[t + static_cast<size_t>(c) * s.ldb] : 0.0f;
+        });
         __syncthreads();
         FactoredSolve(s, sh);
-        if (t < n) {
-            for (int c = 0; c < nrhs; ++c) {
-                sh.x[t][c] = static_cast<double>(sh.bs[t][c]);
-            }
-        }
+        dev::StaticFor<0, kMaxRhs>([&](auto cc) {
+            constexpr int c = decltype(cc)::value;
+            sh.x[t][c] = (t < n && c < nrhs) ? static_cast<double>(sh.bs[t][c]) : 0.0;
+        });
         __syncthreads();
         int iter = 0;
-        double worst = 0.0;
+        double worst = 0.0, prev = 0.0;
+        bool converged = false;
+        double r[kMaxRhs];
         for (;;) {
-            // r = b - A x in FP64 (kept in registers for the correction solve);
-            // norms reduced through LDS
-            double r[kMaxRhs];
-            double rs = 0.0;
-            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                r[c] = (t < n && c < nrhs) ? static_cast<double>(s.B[t + static_cast<size_t>(c) * s.ldb]) : 0.0;
-            });
-            if (t < n) {
-                for (int j0 = 0; j0 < n; j0 += 8) {
-                    float av[8];
-                    dev::StaticFor<0, 8>([&](auto jc) {
-                        constexpr int j = decltype(jc)::value;
-                        av[j] = j0 + j < n ? s.A[t + static_cast<size_t>(j0 + j) * s.lda] : 0.0f;
-                    });
-                    dev::StaticFor<0, 8>([&](auto jc) {
-                        constexpr int j = decltype(jc)::value;
-                        if (j0 + j < n) {
-                            const double a = static_cast<double>(av[j]);
-                            rs += fabs(a);
-                            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                                constexpr int c = decltype(cc)::value;
-                                if (c < nrhs) {
-                                    r[c] = fma(-a, sh.x[j0 + j][c], r[c]);
-                                }
-                            });
+            worst = Residual(s, sh, iter == 0, r);
+            if (iter > 0 && !(worst <= prev)) {
+                // the last correction made things worse: take it back (bs still holds it)
+                if (t < n) {
+                    dev::StaticFor<0, kMaxRhs>([&](auto cc) {
+                        constexpr int c = decltype(cc)::value;
+                        if (c < nrhs) {
+                            sh.x[t][c] -= static_cast<double>(sh.bs[t][c]);
                         }
                     });
                 }
+                worst = prev;
+                __syncthreads();
+                break;
             }
-            const int lane = dev::LaneId();
-            const int wave = t / kWave;
-            {
-                const double m = WaveMaxD(rs);
-                if (lane == 0) {
-                    sh.anorm[wave] = m;
-                }
-            }
-            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                if (c < nrhs) {
-                    const double rv = WaveMaxD(fabs(r[c]));
-                    const double xv = WaveMaxD(t < n ? fabs(sh.x[t][c]) : 0.0);
-                    const double bv = WaveMaxD(t < n ? fabs(static_cast<double>(s.B[t + static_cast<size_t>(c) *
-                                                                                      s.ldb])) : 0.0);
-                    if (lane == 0) {
-                        sh.red[0][wave][c] = rv;
-                        sh.red[1][wave][c] = xv;
-                        sh.red[2][wave][c] = bv;
-                    }
-                }
-            });
-            __syncthreads();
-            if (t < nrhs) {
-                double an = 0.0, rn = 0.0, xn = 0.0, bn = 0.0;
-                for (int q = 0; q < kWaves; ++q) {
-                    an = sh.anorm[q] > an ? sh.anorm[q] : an;
-                    rn = sh.red[0][q][t] > rn ? sh.red[0][q][t] : rn;
-                    xn = sh.red[1][q][t] > xn ? sh.red[1][q][t] : xn;
-                    bn = sh.red[2][q][t] > bn ? sh.red[2][q][t] : bn;
-                }
-                const double den = an * xn + bn;
-                sh.berr[t] = den > 0.0 ? rn / den : (rn > 0.0 ? __builtin_inf() : 0.0);
-            }
-            __syncthreads();
-            worst = 0.0;
-            for (int c = 0; c < nrhs; ++c) {
-                const double be = sh.berr[c];
-                worst = (be > worst || !(be == be)) ? be : worst;
-            }
-            __syncthreads();
-            if (worst <= tol || iter >= maxIter || !(worst == worst)) {
+            converged = worst <= tol;
+            const bool stalled = !(worst == worst) || iter >= maxIter || (iter > 0 && worst > 0.5 * prev);
+            if (converged || stalled) {
                 break;
             }
             // d = LU \ r (FP32), x += d
             dev::StaticFor<0, kMaxRhs>([&](auto cc) {
                 constexpr int c = decltype(cc)::value;
-                if (t < n && c < nrhs) {
-                    sh.bs[t][c] = static_cast<float>(r[c]);
-                }
+                sh.bs[t][c] = static_cast<float>(r[c]);
             });
             __syncthreads();
             FactoredSolve(s, sh);
-            if (t < n) {
-                for (int c = 0; c < nrhs; ++c) {
+            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
+                constexpr int c = decltype(cc)::value;
+                if (t < n && c < nrhs) {
                     sh.x[t][c] += static_cast<double>(sh.bs[t][c]);
                 }
+            });
+            __syncthreads();
+            prev = worst;
+            ++iter;
+        }
+        if (!converged && mayMark != 0) {
+            if (t == 0) {
+                s.state[1] = 1;
             }
             __syncthreads();
-            ++iter;
+            continue;
         }
         if (t < n) {
             for (int c = 0; c < nrhs; ++c) {
@@ -742,6 +899,8 @@ __global__ __launch_bounds__(kThreads, 3) void GesvRefineKernel(const MixedSyste
             if (iterOut != nullptr) iterOut[s.index] = iter;
             if (berrOut != nullptr) berrOut[s.index] = static_cast<float>(worst);
             if (infoOut != nullptr) infoOut[s.index] = 0;
+            if (fallbackOut != nullptr) fallbackOut[s.index] = onlyMarked;
+            s.state[1] = 0;
         }
         __syncthreads();
     }
@@ -768,10 +927,13 @@ size_t RowMapInts(int n)
 
 aclblasStatus_t ValidateMixed(int batch, const int* nA, const int* rA, const float* const* A, const int* ldaA,
                               const float* const* B, const int* ldbA, float* const* X, const int* ldxA, double tol,
-                              int maxIter)
+                              int maxIter, int update)
 {
     if (batch < 0 || !(tol >= 0.0) || maxIter < 0) {
         return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (update < ACLBLAS_MIXED_UPDATE_FP32 || update > ACLBLAS_MIXED_UPDATE_BF16X3) {
+        return ACLBLAS_STATUS_INVALID_ENUM;
     }
     if (batch == 0) {
         return ACLBLAS_STATUS_SUCCESS;
@@ -799,22 +961,42 @@ aclblasStatus_t ValidateMixed(int batch, const int* nA, const int* rA, const flo
     return ACLBLAS_STATUS_SUCCESS;
 }
 
+void LaunchFactor(int update, unsigned grid, hipStream_t stream, const MixedSystem* sys, int count, int onlyMarked)
+{
+    switch (update) {
+    case ACLBLAS_MIXED_UPDATE_BF16:
+        hipLaunchKernelGGL(GesvFactorKernel<1>, dim3(grid), dim3(kThreads), 0, stream, sys, count, onlyMarked);
+        break;
+    case ACLBLAS_MIXED_UPDATE_BF16X2:
+        hipLaunchKernelGGL(GesvFactorKernel<2>, dim3(grid), dim3(kThreads), 0, stream, sys, count, onlyMarked);
+        break;
+    case ACLBLAS_MIXED_UPDATE_BF16X3:
+        hipLaunchKernelGGL(GesvFactorKernel<3>, dim3(grid), dim3(kThreads), 0, stream, sys, count, onlyMarked);
+        break;
+    default:
+        hipLaunchKernelGGL(GesvFactorKernel<0>, dim3(grid), dim3(kThreads), 0, stream, sys, count, onlyMarked);
+        break;
+    }
+}
+
 }  // namespace
 }  // namespace aclblas
 
-extern "C" aclblasStatus_t aclblasSgesvBatchedMixed(
+extern "C" aclblasStatus_t aclblasSgesvBatchedMixedEx(
     aclblasHandle_t handle, int batchCount, const int* nArray, const int* nrhsArray, const float* const Aarray[],
     const int* ldaArray, const float* const Barray[], const int* ldbArray, float* const Xarray[],
-    const int* ldxArray, double tol, int maxIter, int* iterArray, float* berrArray, int* infoArray)
+    const int* ldxArray, double tol, int maxIter, aclblasMixedUpdate_t update, int* iterArray, float* berrArray,
+    int* infoArray, int* fallbackArray)
 {
     using namespace aclblas;
     if (handle == nullptr) {
         return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
     }
+    const int mode = static_cast<int>(update);
     aclblasStatus_t st = ValidateMixed(batchCount, nArray, nrhsArray, Aarray, ldaArray, Barray, ldbArray, Xarray,
-                                       ldxArray, tol, maxIter);
+                                       ldxArray, tol, maxIter, mode);
     if (st != ACLBLAS_STATUS_SUCCESS) {
-        LogError("aclblasSgesvBatchedMixed", "invalid argument (status %d)", static_cast<int>(st));
+        LogError("aclblasSgesvBatchedMixedEx", "invalid argument (status %d)", static_cast<int>(st));
         return st;
     }
     if (batchCount == 0) {
@@ -823,7 +1005,7 @@ extern "C" aclblasStatus_t aclblasSgesvBatchedMixed(
     if (!HandleHasDevice(handle)) {
         return ACLBLAS_STATUS_EXECUTION_FAILED;
     }
-    // largest systems first; workspace: table, then per system LU, row maps, status
+    // largest systems first; workspace: table, then per system LU, inverses, row maps, state
     std::vector<int> order(static_cast<size_t>(batchCount));
     for (int i = 0; i < batchCount; ++i) {
         order[static_cast<size_t>(i)] = i;
@@ -834,8 +1016,10 @@ extern "C" aclblasStatus_t aclblasSgesvBatchedMixed(
     size_t total = tableBytes;
     for (int i = 0; i < batchCount; ++i) {
         const int n = nArray[i];
+        const size_t panels = static_cast<size_t>((n + kNb - 1) / kNb);
         offs[static_cast<size_t>(i)] = total;
-        total += AlignUp(sizeof(float) * static_cast<size_t>(n) * n) + AlignUp(sizeof(int) * RowMapInts(n)) + kAlign;
+        total += AlignUp(sizeof(float) * static_cast<size_t>(n) * n) + AlignUp(sizeof(float) * kInvFloats * panels) +
+                 AlignUp(sizeof(int) * RowMapInts(n)) + kAlign;
     }
     st = EnsureWorkspace(handle, total);
     if (st != ACLBLAS_STATUS_SUCCESS) {
@@ -846,6 +1030,7 @@ extern "C" aclblasStatus_t aclblasSgesvBatchedMixed(
     for (int q = 0; q < batchCount; ++q) {
         const int i = order[static_cast<size_t>(q)];
         const int n = nArray[i];
+        const size_t panels = static_cast<size_t>((n + kNb - 1) / kNb);
         MixedSystem& s = table[static_cast<size_t>(q)];
         char* p = base + offs[static_cast<size_t>(i)];
         s.A = Aarray[i];
@@ -853,9 +1038,11 @@ extern "C" aclblasStatus_t aclblasSgesvBatchedMixed(
         s.X = Xarray[i];
         s.LU = reinterpret_cast<float*>(p);
         p += AlignUp(sizeof(float) * static_cast<size_t>(n) * n);
+        s.dinv = reinterpret_cast<float*>(p);
+        p += AlignUp(sizeof(float) * kInvFloats * panels);
         s.rowmap = reinterpret_cast<int*>(p);
         p += AlignUp(sizeof(int) * RowMapInts(n));
-        s.status = reinterpret_cast<int*>(p);
+        s.state = reinterpret_cast<int*>(p);
         s.n = n;
         s.nrhs = nrhsArray[i];
         s.lda = ldaArray[i];
@@ -875,8 +1062,33 @@ extern "C" aclblasStatus_t aclblasSgesvBatchedMixed(
     const unsigned gridF = static_cast<unsigned>(std::min(batchCount, c.a * cus));
     const unsigned gridS = static_cast<unsigned>(std::min(batchCount, c.b * cus));
     const MixedSystem* sys = static_cast<const MixedSystem*>(dev);
-    hipLaunchKernelGGL(GesvFactorKernel, dim3(gridF), dim3(kThreads), 0, handle->stream, sys, batchCount);
-    hipLaunchKernelGGL(GesvRefineKernel, dim3(gridS), dim3(kThreads), 0, handle->stream, sys, batchCount, tol,
-                       maxIter, iterArray, berrArray, infoArray);
-    return LaunchStatus("aclblasSgesvBatchedMixed");
+    hipStream_t stream = handle->stream;
+    const int mayMark = mode != ACLBLAS_MIXED_UPDATE_FP32 ? 1 : 0;
+    MarkLaunch(handle, 0);
+    LaunchFactor(mode, gridF, stream, sys, batchCount, 0);
+    MarkLaunch(handle, 1);
+    MarkLaunch(handle, 2);
+    hipLaunchKernelGGL(GesvRefineKernel, dim3(gridS), dim3(kThreads), 0, stream, sys, batchCount, tol, maxIter,
+                       mayMark, 0, iterArray, berrArray, infoArray, fallbackArray);
+    MarkLaunch(handle, 3);
+    if (mayMark != 0) {
+        MarkLaunch(handle, 4);
+        LaunchFactor(ACLBLAS_MIXED_UPDATE_FP32, gridF, stream, sys, batchCount, 1);
+        MarkLaunch(handle, 5);
+        MarkLaunch(handle, 6);
+        hipLaunchKernelGGL(GesvRefineKernel, dim3(gridS), dim3(kThreads), 0, stream, sys, batchCount, tol, maxIter,
+                           0, 1, iterArray, berrArray, infoArray, fallbackArray);
+        MarkLaunch(handle, 7);
+    }
+    return LaunchStatus("aclblasSgesvBatchedMixedEx");
+}
+
+extern "C" aclblasStatus_t aclblasSgesvBatchedMixed(
+    aclblasHandle_t handle, int batchCount, const int* nArray, const int* nrhsArray, const float* const Aarray[],
+    const int* ldaArray, const float* const Barray[], const int* ldbArray, float* const Xarray[],
+    const int* ldxArray, double tol, int maxIter, int* iterArray, float* berrArray, int* infoArray)
+{
+    return aclblasSgesvBatchedMixedEx(handle, batchCount, nArray, nrhsArray, Aarray, ldaArray, Barray, ldbArray,
+                                      Xarray, ldxArray, tol, maxIter, ACLBLAS_MIXED_UPDATE_BF16X2, iterArray,
+                                      berrArray, infoArray, nullptr);
 }

@@ -18,13 +18,12 @@ def lu_batch(batch, n, nrhs, seed=0, shift=4.0):
 
 
 def conditioned_systems(count, seed=0, nmin=16, nmax=256, rmin=1, rmax=16, log_cond=(2.0, 6.0)):
-    """Mixed-order systems A = U diag(s) V^T (U, V orthogonal from the QR of Gaussian matrices,
-    computed in FP64; s_i = 10^(-c i / (n-1)) with c = log10(cond) uniform in log_cond) and
-    B ~ N(0,1) with nrhs uniform in [rmin, rmax].
+    """Mixed-order systems A = U diag(s) V^T and B ~ N(0,1): for every system its own U and V, the Q
+    factors of the QR of two Gaussian n x n matrices (FP64); s_i = 10^(-c i / (n-1)) with
+    c = log10(cond) uniform in log_cond; nrhs uniform in [rmin, rmax].
 
     Returns a list of dicts {n, nrhs, cond, A (n*n float32, column-major), B (n*nrhs float32)}
-    in generation order.  Systems of one order share one pair (U, V) and differ in their
-    singular values (one QR pair per order keeps the generation of 10000 systems to seconds).
+    in generation order.  The QR factorisations of one order run as one stacked numpy call.
     """
     rng = np.random.default_rng(seed)
     ns = rng.integers(nmin, nmax + 1, size=count)
@@ -34,11 +33,12 @@ def conditioned_systems(count, seed=0, nmin=16, nmax=256, rmin=1, rmax=16, log_c
     for n in np.unique(ns):
         members = np.nonzero(ns == n)[0]
         n = int(n)
-        U, _ = np.linalg.qr(rng.standard_normal((n, n)))
-        V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        k = len(members)
+        U, _ = np.linalg.qr(rng.standard_normal((k, n, n)))
+        V, _ = np.linalg.qr(rng.standard_normal((k, n, n)))
         expo = np.arange(n) / max(n - 1, 1)
         s = 10.0 ** (-cs[members][:, None] * expo[None, :])            # (k, n)
-        A = (U[None, :, :] * s[:, None, :]) @ V.T                       # row-major (k, n, n)
+        A = (U * s[:, None, :]) @ np.swapaxes(V, 1, 2)                 # row-major (k, n, n)
         for t, idx in enumerate(members):
             r = int(rs[idx])
             Bm = rng.standard_normal((r, n)).astype(np.float32)          # column-major n x r

@@ -397,6 +397,23 @@ def test_gesv_mixed_argument_checks(cpu_handle):
     assert f(cpu_handle, [257], [1], [D], [257], [D], [257], [D], [257], 1e-6, 3) == ns
     assert f(cpu_handle, [4], [17], [D], [4], [D], [4], [D], [4], 1e-6, 3) == ns
     assert f(cpu_handle, [], [], [], [], [], [], [], [], 1e-6, 3) == status_code("SUCCESS")
+    g = B.sgesv_batched_mixed_ex
+    for mode in B.MIXED_UPDATE.values():
+        assert g(cpu_handle, [4], [1], [D], [3], [D], [4], [D], [4], 1e-6, 3, mode) == inv      # lda
+    assert g(cpu_handle, [4], [1], [D], [4], [D], [4], [D], [4], 1e-6, 3, 4) == status_code("INVALID_ENUM")
+    assert g(cpu_handle, [4], [1], [D], [4], [D], [4], [D], [4], 1e-6, 3, -1) == status_code("INVALID_ENUM")
+    assert g(None, [4], [1], [D], [4], [D], [4], [D], [4], 1e-6, 3, 2) == status_code("HANDLE_IS_NULLPTR")
+    assert g(cpu_handle, [], [], [], [], [], [], [], [], 1e-6, 3, 0) == status_code("SUCCESS")
+
+
+def test_set_launch_events_argument_checks(cpu_handle):
+    import ctypes
+    assert B.set_launch_events(None, None, 0) == status_code("HANDLE_IS_NULLPTR")
+    assert B.set_launch_events(cpu_handle, None, -1) == status_code("INVALID_VALUE")
+    assert B.set_launch_events(cpu_handle, None, 2) == status_code("INVALID_VALUE")
+    arr = (ctypes.c_void_p * 2)(None, None)
+    assert B.set_launch_events(cpu_handle, arr, 2) == status_code("SUCCESS")
+    assert B.set_launch_events(cpu_handle, None, 0) == status_code("SUCCESS")
 
 
 def test_gemm_ex_scaled_and_quantizer_argument_checks(cpu_handle):
@@ -421,3 +438,39 @@ def test_gemm_ex_scaled_and_quantizer_argument_checks(cpu_handle):
     assert q(cpu_handle, -1, D, e4, D, D) == inv
     assert q(cpu_handle, 4, D, B.ACL_BF16, D, D) == inv
     assert q(cpu_handle, 4, D, e4, D, None) == inv
+
+
+STRSM_ROWS = [r for r in load_cases("strsm.csv") if status_code(r["expect_result"]) != 0]
+
+
+@pytest.mark.parametrize("row", STRSM_ROWS, ids=[r["case_name"] for r in STRSM_ROWS])
+def test_strsm_argument_rows(cpu_handle, row):
+    """The reference table's error rows (test/trsm/strsm/arch35/strsm_test.csv) on a handle without a device."""
+    side = B.SIDE.get(row["side"], None) or int(row["side"])
+    uplo = B.FILL.get(row["uplo"], None) or int(row["uplo"])
+    trans = {"N": B.OP_N, "T": B.OP_T, "C": B.OP_C}.get(row["trans"], None) or int(row["trans"])
+    diag = B.DIAG.get(row["diag"], None) or int(row["diag"])
+    st = B.strsm(cpu_handle, side, uplo, trans, diag, int(row["m"]), int(row["n"]), float(row["alpha"]), DUMMY,
+                 int(row["lda"]), DUMMY, int(row["ldb"]))
+    assert st == status_code(row["expect_result"])
+
+
+def test_strsm_argument_order(cpu_handle):
+    """Check order of reference blas/trsm/arch35/strsm_host.cpp:406-452."""
+    f = B.strsm
+    L, R = B.SIDE["LEFT"], B.SIDE["RIGHT"]
+    lo, up = B.FILL["LOWER"], B.FILL["UPPER"]
+    nu = B.DIAG["NON_UNIT"]
+    inv = status_code("INVALID_VALUE")
+    D = DUMMY
+    assert f(None, 999, lo, B.OP_N, nu, 4, 4, 1.0, D, 4, D, 4) == status_code("HANDLE_IS_NULLPTR")
+    assert f(cpu_handle, 999, lo, B.OP_N, nu, -1, 4, 1.0, D, 4, D, 4) == inv          # side before m
+    assert f(cpu_handle, L, 999, B.OP_N, nu, 0, 4, 1.0, D, 4, D, 4) == inv            # uplo before quick return
+    assert f(cpu_handle, L, lo, B.OP_N, nu, 0, 4, None, D, 4, D, 4) == inv            # alpha before quick return
+    assert f(cpu_handle, L, lo, B.OP_N, nu, 0, 4, 1.0, None, 0, None, 0) == status_code("SUCCESS")
+    assert f(cpu_handle, L, lo, B.OP_N, nu, 4, 4, 1.0, D, 4, None, 4) == inv          # B
+    assert f(cpu_handle, L, lo, B.OP_N, nu, 4, 4, 1.0, None, 4, D, 4) == inv          # A when alpha != 0
+    assert f(cpu_handle, R, up, B.OP_T, nu, 8, 4, 1.0, D, 3, D, 8) == inv             # right: lda >= n
+    assert f(cpu_handle, R, up, B.OP_T, nu, 8, 4, 1.0, D, 4, D, 8) == status_code("EXECUTION_FAILED")
+    assert f(cpu_handle, L, lo, B.OP_C, B.DIAG["UNIT"], 4, 4, 0.0, None, 4, D, 4) == \
+        status_code("EXECUTION_FAILED")   # alpha = 0 needs no A; no device here

@@ -210,3 +210,64 @@ def test_matinv_graph_replays():
     finally:
         torch.cuda.synchronize()
         h.destroy()
+
+
+def test_graph_replays_after_workspace_growth():
+    """A graph that captured the mixed solve (its table and factors live in the handle's arena) keeps working
+    after a later eager call grew the arena: the captured arena is kept until aclblasDestroy, so the replays
+    touch no freed memory and still equal eager calls bit for bit."""
+    dev = torch.device("cuda")
+    stream = torch.cuda.Stream()
+    h = B.Handle()
+    assert h.set_stream(stream.cuda_stream) == 0
+    try:
+        systems = W.conditioned_systems(24, seed=31)
+        dA, dB, offA, offB = _pack(systems, dev)
+        dX = torch.zeros_like(dB)
+        ns = [s["n"] for s in systems]
+        rs = [s["nrhs"] for s in systems]
+        pa = [dA.data_ptr() + 4 * o for o in offA]
+        pb = [dB.data_ptr() + 4 * o for o in offB]
+        px = [dX.data_ptr() + 4 * o for o in offB]
+        iters = torch.zeros(len(systems), dtype=torch.int32, device=dev)
+
+        def solve(xptrs, it, count=len(systems)):
+            return B.sgesv_batched_mixed(h.value, ns[:count], rs[:count], pa[:count], ns[:count], pb[:count],
+                                         ns[:count], xptrs[:count], ns[:count], 1e-9, 8, it.data_ptr(), None, None)
+
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            assert solve(px, iters) == 0
+        stream.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream, capture_error_mode="relaxed"):
+            assert solve(px, iters) == 0
+        # an eager call whose workspace needs exceed the arena: 400 systems of order 256 (about 110 MiB of
+        # factors) make the handle replace its 32 MiB arena
+        big = W.conditioned_systems(400, seed=32, nmin=256, nmax=256, rmin=1, rmax=1)
+        bA, bB, boA, boB = _pack(big, dev)
+        bX = torch.zeros_like(bB)
+        bn = [256] * len(big)
+        br = [1] * len(big)
+        with torch.cuda.stream(stream):
+            assert B.sgesv_batched_mixed(h.value, bn, br, [bA.data_ptr() + 4 * o for o in boA], bn,
+                                         [bB.data_ptr() + 4 * o for o in boB], bn,
+                                         [bX.data_ptr() + 4 * o for o in boB], bn, 1e-6, 4, None, None, None) == 0
+        stream.synchronize()
+        dX2 = torch.zeros_like(dB)
+        px2 = [dX2.data_ptr() + 4 * o for o in offB]
+        it2 = torch.zeros_like(iters)
+        rng = np.random.default_rng(6)
+        for rep in range(3):
+            newB = torch.from_numpy(rng.standard_normal(dB.numel()).astype(np.float32)).to(dev)
+            with torch.cuda.stream(stream):
+                dB.copy_(newB)
+                dX.zero_()
+                g.replay()
+                assert solve(px2, it2) == 0
+            stream.synchronize()
+            assert torch.equal(dX, dX2), "replay %d after the arena grew differs from the eager call" % rep
+            assert torch.equal(iters, it2)
+    finally:
+        torch.cuda.synchronize()
+        h.destroy()
