@@ -671,6 +671,140 @@ double orc_time_getrf_getrs_mixed(const float* A, const long long* offA, const f
 int orc_hardware_threads() { return static_cast<int>(std::thread::hardware_concurrency()); }
 
 // ---------------------------------------------------------------------------
+// Triangular solve with multiple right-hand sides (FP32), the golden of the
+// reference's aclblasStrsm tests: test/trsm/strsm/strsm_golden.h:19-36 calls
+// cblas_strsm (netlib reference BLAS, column-major).  This restates the loop
+// order of netlib strsm.f for every side / uplo / trans / diag case: left side
+// by columns of B with division by the diagonal, right side by columns of B
+// with multiplication by the diagonal's reciprocal, alpha applied where
+// strsm.f applies it, and every multiply-subtract rounded separately.
+// side: 0 left, 1 right; upper: 1 upper; trans: 0 N, 1 T (C = T); unit: 1 unit.
+// ---------------------------------------------------------------------------
+void orc_trsm(int side, int upper, int trans, int unit, int m, int n, float alpha, const float* A, int lda, float* B,
+              int ldb)
+{
+    auto a = [&](int i, int j) -> float { return A[i + static_cast<size_t>(j) * lda]; };
+    auto b = [&](int i, int j) -> float& { return B[i + static_cast<size_t>(j) * ldb]; };
+    if (m == 0 || n == 0) {
+        return;
+    }
+    if (alpha == 0.0f) {
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < m; ++i) b(i, j) = 0.0f;
+        return;
+    }
+    if (side == 0) {
+        if (!trans) {
+            for (int j = 0; j < n; ++j) {
+                if (alpha != 1.0f)
+                    for (int i = 0; i < m; ++i) b(i, j) = alpha * b(i, j);
+                if (upper) {
+                    for (int k = m - 1; k >= 0; --k) {
+                        if (b(k, j) != 0.0f) {
+                            if (!unit) b(k, j) = b(k, j) / a(k, k);
+                            const float bk = b(k, j);
+                            for (int i = 0; i < k; ++i) b(i, j) = b(i, j) - bk * a(i, k);
+                        }
+                    }
+                } else {
+                    for (int k = 0; k < m; ++k) {
+                        if (b(k, j) != 0.0f) {
+                            if (!unit) b(k, j) = b(k, j) / a(k, k);
+                            const float bk = b(k, j);
+                            for (int i = k + 1; i < m; ++i) b(i, j) = b(i, j) - bk * a(i, k);
+                        }
+                    }
+                }
+            }
+        } else {
+            for (int j = 0; j < n; ++j) {
+                if (upper) {
+                    for (int i = 0; i < m; ++i) {
+                        float t = alpha * b(i, j);
+                        for (int k = 0; k < i; ++k) t = t - a(k, i) * b(k, j);
+                        if (!unit) t = t / a(i, i);
+                        b(i, j) = t;
+                    }
+                } else {
+                    for (int i = m - 1; i >= 0; --i) {
+                        float t = alpha * b(i, j);
+                        for (int k = i + 1; k < m; ++k) t = t - a(k, i) * b(k, j);
+                        if (!unit) t = t / a(i, i);
+                        b(i, j) = t;
+                    }
+                }
+            }
+        }
+        return;
+    }
+    if (!trans) {
+        if (upper) {
+            for (int j = 0; j < n; ++j) {
+                if (alpha != 1.0f)
+                    for (int i = 0; i < m; ++i) b(i, j) = alpha * b(i, j);
+                for (int k = 0; k < j; ++k) {
+                    if (a(k, j) != 0.0f) {
+                        const float akj = a(k, j);
+                        for (int i = 0; i < m; ++i) b(i, j) = b(i, j) - akj * b(i, k);
+                    }
+                }
+                if (!unit) {
+                    const float t = 1.0f / a(j, j);
+                    for (int i = 0; i < m; ++i) b(i, j) = t * b(i, j);
+                }
+            }
+        } else {
+            for (int j = n - 1; j >= 0; --j) {
+                if (alpha != 1.0f)
+                    for (int i = 0; i < m; ++i) b(i, j) = alpha * b(i, j);
+                for (int k = j + 1; k < n; ++k) {
+                    if (a(k, j) != 0.0f) {
+                        const float akj = a(k, j);
+                        for (int i = 0; i < m; ++i) b(i, j) = b(i, j) - akj * b(i, k);
+                    }
+                }
+                if (!unit) {
+                    const float t = 1.0f / a(j, j);
+                    for (int i = 0; i < m; ++i) b(i, j) = t * b(i, j);
+                }
+            }
+        }
+    } else {
+        if (upper) {
+            for (int k = n - 1; k >= 0; --k) {
+                if (!unit) {
+                    const float t = 1.0f / a(k, k);
+                    for (int i = 0; i < m; ++i) b(i, k) = t * b(i, k);
+                }
+                for (int j = 0; j < k; ++j) {
+                    if (a(j, k) != 0.0f) {
+                        const float t = a(j, k);
+                        for (int i = 0; i < m; ++i) b(i, j) = b(i, j) - t * b(i, k);
+                    }
+                }
+                if (alpha != 1.0f)
+                    for (int i = 0; i < m; ++i) b(i, k) = alpha * b(i, k);
+            }
+        } else {
+            for (int k = 0; k < n; ++k) {
+                if (!unit) {
+                    const float t = 1.0f / a(k, k);
+                    for (int i = 0; i < m; ++i) b(i, k) = t * b(i, k);
+                }
+                for (int j = k + 1; j < n; ++j) {
+                    if (a(j, k) != 0.0f) {
+                        const float t = a(j, k);
+                        for (int i = 0; i < m; ++i) b(i, j) = b(i, j) - t * b(i, k);
+                    }
+                }
+                if (alpha != 1.0f)
+                    for (int i = 0; i < m; ++i) b(i, k) = alpha * b(i, k);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // GEMM
 // ---------------------------------------------------------------------------
 

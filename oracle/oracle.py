@@ -58,6 +58,8 @@ def lib():
                                         c.c_int, _i32p, c.c_int, c.c_int]
         L.orc_time_getrf_getrs.argtypes = [_f32p, _f32p, c.c_int, c.c_int, c.c_int, c.c_int, c.c_int]
         L.orc_time_getrf_getrs.restype = c.c_double
+        L.orc_trsm.argtypes = [c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_float, _f32p, c.c_int,
+                               _f32p, c.c_int]
         L.orc_time_getrf_getrs_mixed.argtypes = [_f32p, _i64p, _f32p, _i64p, _i32p, _i32p, c.c_int, c.c_int,
                                                  c.c_int]
         L.orc_time_getrf_getrs_mixed.restype = c.c_double
@@ -204,6 +206,14 @@ def time_getrf_getrs_mixed(A, offA, B, offB, n, nrhs, nthreads, reps=3):
     n = np.ascontiguousarray(n, dtype=np.int32)
     nrhs = np.ascontiguousarray(nrhs, dtype=np.int32)
     return float(lib().orc_time_getrf_getrs_mixed(A, offA, B, offB, n, nrhs, len(n), nthreads, reps))
+
+
+def trsm(side, upper, trans, unit, m, n, alpha, A, lda, B, ldb):
+    """B (numpy float32, modified in place) := alpha op(A)^-1 B (side 0) or alpha B op(A)^-1 (side 1), netlib
+    strsm loop order in FP32."""
+    lib().orc_trsm(int(side), int(upper), int(trans), int(unit), m, n, float(alpha),
+                   np.ascontiguousarray(A, dtype=np.float32), lda, B, ldb)
+    return B
 
 
 # ---------------------------------------------------------------- GEMM

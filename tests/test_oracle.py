@@ -314,3 +314,78 @@ def test_backward_error_oracle():
     X = B.copy()
     X[0, 0, 2] = 3.5   # residual 0.5, ||A||=1, ||x||=3.5, ||b||=3
     assert abs(O.backward_error(A, B, X)[0] - 0.5 / (3.5 + 3.0)) < 1e-12
+
+
+# ------------------------------------------------------------------ triangular solve (aclblasStrsm golden)
+def _tri(m, upper, seed, boost=True):
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(-1, 1, (m, m)).astype(np.float32)
+    A = np.triu(A) if upper else np.tril(A)
+    if boost:
+        d = np.diag(A).copy()
+        A[np.arange(m), np.arange(m)] = d + np.where(d >= 0, 1, -1) * max(5.0, m)
+    return A
+
+
+@pytest.mark.parametrize("side", [0, 1])
+@pytest.mark.parametrize("upper", [0, 1])
+@pytest.mark.parametrize("trans", [0, 1])
+@pytest.mark.parametrize("unit", [0, 1])
+def test_oracle_trsm_matches_fp64_and_blas(side, upper, trans, unit):
+    """orc_trsm against an FP64 triangular solve (FP32-level agreement) and scipy's BLAS strsm (another FP32
+    summation order), every side / uplo / trans / diag case, alpha != 1."""
+    import scipy.linalg as sl
+    import scipy.linalg.blas as sb
+    m, n = 37, 29
+    k = m if side == 0 else n
+    A = _tri(k, upper, 10 + 8 * side + 4 * upper + 2 * trans + unit)
+    lda = k + 3
+    Ap = np.zeros((k, lda), np.float32)          # column-major with padding: Ap[j, i] = A(i, j)
+    Ap[:, :k] = A.T
+    rng = np.random.default_rng(99)
+    Bm = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+    ldb = m + 2
+    Bp = np.zeros((n, ldb), np.float32)
+    Bp[:, :m] = Bm.T
+    alpha = 0.75
+    out = Bp.copy().reshape(-1)
+    O.trsm(side, upper, trans, unit, m, n, alpha, Ap.reshape(-1), lda, out, ldb)
+    X = out.reshape(n, ldb)[:, :m].T.astype(np.float64)
+    assert np.array_equal(out.reshape(n, ldb)[:, m:], Bp[:, m:])          # padding untouched
+    A64 = A.astype(np.float64)
+    if unit:
+        A64[np.arange(k), np.arange(k)] = 1.0
+    opA = A64.T if trans else A64
+    ref = np.linalg.solve(opA, alpha * Bm.astype(np.float64)) if side == 0 else \
+        np.linalg.solve(opA.T, alpha * Bm.astype(np.float64).T).T
+    assert np.abs(X - ref).max() <= 1e-5 * max(1.0, np.abs(ref).max())
+    blas = sb.strsm(alpha, A, Bm, side=side, lower=1 - upper, trans_a=trans, diag=unit)
+    assert np.abs(X - blas).max() <= 1e-5 * max(1.0, np.abs(blas).max())
+    # the check separates the cases: the solution with the operand transposed is far off
+    X2 = np.linalg.solve(opA.T, alpha * Bm.astype(np.float64)) if side == 0 else \
+        np.linalg.solve(opA, alpha * Bm.astype(np.float64).T).T
+    assert np.abs(X - X2).max() > 1e-3
+
+
+def test_oracle_trsm_known_answers():
+    """Identity A: B scaled by alpha exactly; diagonal A (right side, N): column j divided through the reciprocal
+    of a_jj as strsm.f does; alpha = 0: B zeroed, NaN included."""
+    m, n = 5, 4
+    B0 = np.arange(1, m * n + 1, dtype=np.float32)
+    I = np.eye(m, dtype=np.float32).reshape(-1)
+    for side, k in ((0, m), (1, n)):
+        Ik = np.eye(k, dtype=np.float32).reshape(-1)
+        for upper in (0, 1):
+            for trans in (0, 1):
+                out = B0.copy()
+                O.trsm(side, upper, trans, 0, m, n, 2.0, Ik, k, out, m)
+                assert np.array_equal(out, 2 * B0)
+    d = np.array([2.0, 4.0, 8.0, 3.0], np.float32)
+    out = B0.copy()
+    O.trsm(1, 1, 0, 0, m, n, 1.0, np.diag(d).T.reshape(-1).copy(), n, out, m)
+    expect = (B0.reshape(n, m) * (np.float32(1.0) / d)[:, None]).reshape(-1)
+    assert np.array_equal(out, expect)
+    out = B0.copy()
+    out[3] = np.nan
+    O.trsm(0, 0, 0, 0, m, n, 0.0, I, m, out, m)
+    assert not out.any()
