@@ -119,6 +119,17 @@ aclblasStatus_t aclblasGemmExScaled(
 aclblasStatus_t aclblasQuantizeFp8(aclblasHandle_t handle, long long count, const float* x, aclDataType qtype,
                                    void* q, float* scale);
 
+/* No counterpart in the reference.  aclblasQuantizeFp8 for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the per-block maxima live in a 4 KiB device array of the library
+ * (one per GPU, held until the process ends) instead of the handle's workspace,
+ * so the call allocates nothing and never grows the workspace.  Same arguments,
+ * checks, results and stream semantics as aclblasQuantizeFp8; any count.  All
+ * calls on one GPU share that array: calls that overlap on one GPU (several
+ * streams or host threads) race. */
+aclblasStatus_t aclblasQuantizeFp8NoAlloc(aclblasHandle_t handle, long long count, const float* x,
+                                          aclDataType qtype, void* q, float* scale);
+
 /* reference include/cann_ops_blas.h:409-414.  C[i] = alpha*op(A[i])*op(B[i]) +
  * beta*C[i] over device arrays of device pointers, one shape for the batch.
  * Accepted types: FP16 (COMPUTE_16F or 32F), BF16 and FP8 E4M3/E5M2 (any mix,

@@ -70,6 +70,7 @@ _SIGNATURES = {
     "aclblasSgesvBatchedMixedEx": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, _I, _I, _P, _P, _P, _P],
     "aclblasGemmExScaled": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _P, _P, _I, _I, _P, _P, _P, _I, _I],
     "aclblasQuantizeFp8": [_P, ctypes.c_longlong, _P, _I, _P, _P],
+    "aclblasQuantizeFp8NoAlloc": [_P, ctypes.c_longlong, _P, _I, _P, _P],
     "aclblasSetTuningOverride": [_P, _I, _I, _I],
     "aclblasGetTuningChoice": [_P, _I, _I, _P, _P],
     "aclblasTuningBucket": [_P, _I, _I, ctypes.c_longlong, _I, _P],
@@ -278,6 +279,11 @@ def gemm_ex_scaled(h, ta, tb, m, n, k, alpha, A, atype, lda, scale_a, B, btype, 
     """scale_a / scale_b: device addresses of one float each (or None)."""
     return load().aclblasGemmExScaled(h, ta, tb, m, n, k, _fp(alpha), A, atype, lda, scale_a, B, btype, ldb, scale_b,
                                       _fp(beta), C, ctype, ldc)
+
+
+def quantize_fp8_no_alloc(h, count, x, qtype, q, scale):
+    """aclblasQuantizeFp8NoAlloc: as quantize_fp8, with the library's own scratch array (allocates nothing)."""
+    return load().aclblasQuantizeFp8NoAlloc(h, count, x, qtype, q, scale)
 
 
 def quantize_fp8(h, count, x, qtype, q, scale):

@@ -17,6 +17,8 @@
 // global atomics or zeroed counters are needed), quantises its part, and block
 // 0 stores the scale.  Aligned operands move as float4 loads and packed
 // 4-byte stores.
+#include <mutex>
+
 #include "cann_ops_blas.h"
 #include "gemm_kernel.h"
 #include "internal.h"
@@ -29,6 +31,36 @@ using namespace gemm;
 
 constexpr int kQThreads = 256;
 constexpr int kQBlocks = 1024;
+constexpr int kMaxDevices = 64;
+
+// Per-block maxima of aclblasQuantizeFp8NoAlloc: 4 KiB on every GPU the
+// library runs on, held until the process ends.  Host code never passes this
+// name to a kernel: it passes the device address that hipGetSymbolAddress
+// returns for the current device (QuantScratch).
+static __device__ __align__(256) float scratch[kQBlocks];
+
+// Device address of `scratch` on the current device, looked up once per
+// device; nullptr if the lookup fails.
+float* QuantScratch(int device)
+{
+    static std::mutex mu;
+    static float* ptrs[kMaxDevices] = {};
+    static bool done[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) {
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!done[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        ptrs[device] = static_cast<float*>(p);
+        done[device] = true;
+    }
+    return ptrs[device];
+}
 
 // float -> FP8 (E4M3FN when E4M3, else E5M2), round to nearest even,
 // saturating to the largest finite value; NaN -> the format's NaN.
@@ -274,10 +306,12 @@ extern "C" aclblasStatus_t aclblasGemmExScaled(
     return LaunchStatus("aclblasGemmExScaled");
 }
 
-extern "C" aclblasStatus_t aclblasQuantizeFp8(aclblasHandle_t handle, long long count, const float* x,
-                                              aclDataType qtype, void* q, float* scale)
+namespace aclblas {
+namespace {
+
+aclblasStatus_t CheckQuantizeArgs(aclblasHandle_t handle, long long count, const float* x, aclDataType qtype,
+                                  const void* q, const float* scale)
 {
-    using namespace aclblas;
     if (handle == nullptr) {
         return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
     }
@@ -290,18 +324,20 @@ extern "C" aclblasStatus_t aclblasQuantizeFp8(aclblasHandle_t handle, long long 
     if (!HandleHasDevice(handle)) {
         return ACLBLAS_STATUS_EXECUTION_FAILED;
     }
-    aclblasStatus_t st = EnsureWorkspace(handle, kQBlocks * sizeof(float));
-    if (st != ACLBLAS_STATUS_SUCCESS) {
-        return st;
-    }
-    float* partial = static_cast<float*>(handle->workspace);
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+// Both launches of the quantiser with the per-block maxima in `partial`
+// (kQBlocks floats of device memory).
+void LaunchQuantizer(hipStream_t stream, long long count, const float* x, aclDataType qtype, void* q, float* scale,
+                     float* partial)
+{
     long long blocks = (count / 4 + kQThreads - 1) / kQThreads;
     blocks = blocks < 1 ? 1 : (blocks > kQBlocks ? kQBlocks : blocks);
     const dim3 grid(static_cast<unsigned>(blocks));
     const bool vec = reinterpret_cast<uintptr_t>(x) % 16 == 0 && reinterpret_cast<uintptr_t>(q) % 4 == 0;
     uint8_t* q8 = static_cast<uint8_t*>(q);
     const int parts = static_cast<int>(blocks);
-    hipStream_t stream = handle->stream;
     if (qtype == ACL_FLOAT8_E4M3FN) {
         if (vec) {
             LaunchQuantize<true, true>(stream, grid, x, count, partial, parts, q8, scale);
@@ -315,5 +351,52 @@ extern "C" aclblasStatus_t aclblasQuantizeFp8(aclblasHandle_t handle, long long 
             LaunchQuantize<false, false>(stream, grid, x, count, partial, parts, q8, scale);
         }
     }
+}
+
+}  // namespace
+}  // namespace aclblas
+
+extern "C" aclblasStatus_t aclblasQuantizeFp8(aclblasHandle_t handle, long long count, const float* x,
+                                              aclDataType qtype, void* q, float* scale)
+{
+    using namespace aclblas;
+    aclblasStatus_t st = CheckQuantizeArgs(handle, count, x, qtype, q, scale);
+    if (st != ACLBLAS_STATUS_SUCCESS) {
+        return st;
+    }
+    st = EnsureWorkspace(handle, kQBlocks * sizeof(float));
+    if (st != ACLBLAS_STATUS_SUCCESS) {
+        return st;
+    }
+    LaunchQuantizer(handle->stream, count, x, qtype, q, scale, static_cast<float*>(handle->workspace));
     return LaunchStatus("aclblasQuantizeFp8");
+}
+
+// The quantiser without the handle's workspace: the per-block maxima live in
+// the file-scope device array `scratch` (kQBlocks floats per GPU), reached
+// through its device address.  Nothing is allocated, so the call can be made
+// where allocation is not allowed.  Every call on one GPU shares that array
+// (the first launch writes every entry the second reads, so nothing needs
+// clearing); calls that overlap on one GPU race.
+extern "C" aclblasStatus_t aclblasQuantizeFp8NoAlloc(aclblasHandle_t handle, long long count, const float* x,
+                                                     aclDataType qtype, void* q, float* scale)
+{
+    using namespace aclblas;
+    aclblasStatus_t st = CheckQuantizeArgs(handle, count, x, qtype, q, scale);
+    if (st != ACLBLAS_STATUS_SUCCESS) {
+        return st;
+    }
+    int device = -1;
+    if (hipGetDevice(&device) != hipSuccess) {
+        (void)hipGetLastError();
+        return ACLBLAS_STATUS_EXECUTION_FAILED;
+    }
+    float* partial = QuantScratch(device);
+    if (partial == nullptr) {
+        LogError("aclblasQuantizeFp8NoAlloc", "device address of the quantiser's scratch array not found on device %d",
+                 device);
+        return ACLBLAS_STATUS_EXECUTION_FAILED;
+    }
+    LaunchQuantizer(handle->stream, count, x, qtype, q, scale, partial);
+    return LaunchStatus("aclblasQuantizeFp8NoAlloc");
 }

@@ -474,3 +474,17 @@ def test_strsm_argument_order(cpu_handle):
     assert f(cpu_handle, R, up, B.OP_T, nu, 8, 4, 1.0, D, 4, D, 8) == status_code("EXECUTION_FAILED")
     assert f(cpu_handle, L, lo, B.OP_C, B.DIAG["UNIT"], 4, 4, 0.0, None, 4, D, 4) == \
         status_code("EXECUTION_FAILED")   # alpha = 0 needs no A; no device here
+
+
+def test_quantize_no_alloc_argument_checks(cpu_handle):
+    """aclblasQuantizeFp8NoAlloc checks its arguments like aclblasQuantizeFp8 (no device here: valid arguments end
+    at EXECUTION_FAILED before any launch)."""
+    q = B.quantize_fp8_no_alloc
+    D = DUMMY
+    e4 = B.DTYPES["FP8_E4M3FN"]
+    assert q(None, 4, D, e4, D, D) == status_code("HANDLE_IS_NULLPTR")
+    assert q(cpu_handle, -1, D, e4, D, D) == status_code("INVALID_VALUE")
+    assert q(cpu_handle, 4, D, B.ACL_BF16, D, D) == status_code("INVALID_VALUE")
+    assert q(cpu_handle, 4, D, e4, D, None) == status_code("INVALID_VALUE")
+    assert q(cpu_handle, 4, None, e4, D, D) == status_code("INVALID_VALUE")
+    assert q(cpu_handle, 4, D, e4, D, D) == status_code("EXECUTION_FAILED")
