@@ -66,25 +66,23 @@
 #include "cann_ops_blas.h"
 #include "device_common.h"
 #include "internal.h"
+#include "lu_blocked.h"
 #include "tuning.h"
 
 namespace aclblas {
 namespace {
 
 using dev::kWave;
+using lub::f32x4;
+using lub::FactorShared;
+using lub::kMaxN;
+using lub::kNb;
+using lub::kThreads;
+using lub::kWaves;
+using lub::PanelView;
 
-constexpr int kMaxN = 256;
 constexpr int kMaxRhs = 16;
-constexpr int kNb = 32;                  // panel width
-constexpr int kThreads = 256;            // four waves
-constexpr int kWaves = kThreads / kWave;
-constexpr int kTileN = 16;               // sweep tile width
-constexpr int kMaxTiles = kMaxN / 16;    // 16-row accumulator tiles per column tile
-constexpr int kLd = kNb + 4;             // row stride (floats) of 32-wide FP32 LDS blocks: 16-byte rows
 constexpr int kInvFloats = kNb * kNb;    // per panel: inv(L11), 32 x 32 column-major
-
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 struct MixedSystem {
     const float* A;
@@ -99,382 +97,6 @@ struct MixedSystem {
 
 __device__ __forceinline__ int RowMapOffset(int n, int p) { return p * n - 16 * p * (p - 1); }
 
-__device__ __forceinline__ uint16_t Bf16Rne(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7FFFFFFFu) > 0x7F800000u) {
-        return static_cast<uint16_t>((u >> 16) | 0x40u);
-    }
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return static_cast<uint16_t>(u >> 16);
-}
-__device__ __forceinline__ float Bf16ToF(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
-
-// x = sum of the P planes: each plane is the BF16 rounding of what the planes
-// before it leave (three planes hold every finite FP32 value of normal range
-// exactly).
-template <int P>
-__device__ __forceinline__ void SplitPlanes(float x, uint16_t (&h)[P])
-{
-    float r = x;
-    dev::StaticFor<0, P>([&](auto pc) {
-        constexpr int p = decltype(pc)::value;
-        h[p] = Bf16Rne(r);
-        if constexpr (p + 1 < P) {
-            r = r - Bf16ToF(h[p]);
-        }
-    });
-}
-
-// -L21 and U12 of the current panel in the operand format of the update.
-template <int P>
-struct UpdateOperands {  // P BF16 planes each
-    alignas(16) uint16_t l[P][kMaxN - kNb][kNb];          // -L21 [plane][row][k]
-    alignas(16) uint16_t u[kWaves][P][kTileN][kNb];       // U12^T per wave [plane][col][k]
-};
-template <>
-struct UpdateOperands<0> {  // FP32
-    alignas(16) float l[kMaxN - kNb][kLd];                // -L21 [row][k]
-    alignas(16) float u[kWaves][kTileN][kLd];             // U12^T per wave [col][k]
-};
-
-template <int P>
-struct FactorShared {
-    alignas(16) float cand[2][kWaves][kNb];               // each wave's best row, by column parity
-    uint32_t candKey[2][kWaves];
-    int candPos[2][kWaves];
-    int rowmap[kMaxN];                                    // panel row map
-    alignas(16) float l11[kNb][kLd];                      // panel rows 0..31: L11 below, U11 on/above the diagonal
-    alignas(16) float linv[kNb][kLd];                     // inv(L11), row-major
-    alignas(16) float ut[kWaves][kNb][kTileN + 1];        // per-wave top rows of the sweep tile
-    UpdateOperands<P> op;
-};
-
-__device__ __forceinline__ void WaveSync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Panel factorisation by the whole workgroup: wave v holds rows 64v..64v+63 of
-// the panel (one row per lane) in registers.  Per column, each wave reduces
-// its candidates with DPP, the wave's best lane publishes its row, |a| key and
-// position in LDS (double-buffered by column parity), one barrier, and every
-// lane picks the same winner among the four.  Returns the zero-pivot column
-// + 1, or 0 (uniform over the workgroup).
-template <int P>
-__device__ __forceinline__ int FactorPanel(const MixedSystem& s, int k0, FactorShared<P>& sh)
-{
-    const int n = s.n;
-    const int lane = dev::LaneId();
-    const int wave = static_cast<int>(threadIdx.x) / kWave;
-    const int rows = n - k0;
-    const int w = min(kNb, rows);
-    const int r = wave * kWave + lane;
-    const bool valid = r < rows;
-    float* LU = s.LU;
-
-    float a[kNb];
-    dev::StaticFor<0, kNb>([&](auto jc) {
-        constexpr int j = decltype(jc)::value;
-        a[j] = (valid && j < w) ? LU[(k0 + r) + static_cast<size_t>(k0 + j) * n] : 0.0f;
-    });
-    int pos = r;
-    int zeroCol = 0;
-
-    dev::StaticFor<0, kNb>([&](auto jc) {
-        constexpr int j = decltype(jc)::value;
-        if (j >= w || zeroCol != 0) {
-            return;
-        }
-        constexpr int buf = j & 1;
-        const bool cand = valid && pos >= j;
-        const uint32_t key = cand ? dev::MagKey(a[j]) : 0u;
-        // wave best: largest key, then lowest position (non-candidates lose)
-        const uint32_t best = dev::SegReduce<kWave, true>(key);
-        const bool tied = cand && key == best;
-        const uint32_t lowest = dev::SegReduce<kWave, false>(tied ? static_cast<uint32_t>(pos) : 0xFFFFu);
-        const bool mine = tied && static_cast<uint32_t>(pos) == lowest;
-        if (mine) {
-            dev::StaticFor<0, kNb / 4>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                *reinterpret_cast<float4*>(&sh.cand[buf][wave][4 * c]) =
-                    make_float4(a[4 * c], a[4 * c + 1], a[4 * c + 2], a[4 * c + 3]);
-            });
-            sh.candKey[buf][wave] = key;
-            sh.candPos[buf][wave] = pos;
-        } else if (lane == 0 && lowest == 0xFFFFu) {
-            sh.candPos[buf][wave] = 0xFFFF;  // no candidate in this wave
-            sh.candKey[buf][wave] = 0;
-        }
-        __syncthreads();
-        int wb = -1;
-        uint32_t bk = 0;
-        int bp = 0xFFFF;
-        dev::StaticFor<0, kWaves>([&](auto vc) {
-            constexpr int v = decltype(vc)::value;
-            const int p = sh.candPos[buf][v];
-            const uint32_t k = sh.candKey[buf][v];
-            if (p != 0xFFFF && (wb < 0 || k > bk || (k == bk && p < bp))) {
-                wb = v;
-                bk = k;
-                bp = p;
-            }
-        });
-        wb = dev::Uniform(wb);
-        const int pw = dev::Uniform(bp);
-        float u[kNb];
-        dev::StaticFor<j / 4, kNb / 4>([&](auto cc) {
-            constexpr int c = decltype(cc)::value;
-            const float4 v = *reinterpret_cast<const float4*>(&sh.cand[buf][wb][4 * c]);
-            u[4 * c] = v.x;
-            u[4 * c + 1] = v.y;
-            u[4 * c + 2] = v.z;
-            u[4 * c + 3] = v.w;
-        });
-        const float pivot = u[j];
-        if (pivot == 0.0f) {
-            zeroCol = k0 + j + 1;
-            return;
-        }
-        const bool winner = pos == pw && cand;
-        if (pos == j) {
-            pos = pw;
-        }
-        if (winner) {
-            pos = j;
-        }
-        const float rcp = 1.0f / pivot;
-        if (valid && pos > j) {
-            const float l = a[j] * rcp;
-            a[j] = l;
-            dev::StaticFor<j + 1, kNb>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                a[c] = fmaf(-l, u[c], a[c]);
-            });
-        }
-    });
-    if (zeroCol != 0) {
-        return zeroCol;
-    }
-
-    // write back in pivoted order; row map; L11/U11 (FP32) and -L21 in the update's format
-    int* gmap = s.rowmap + RowMapOffset(n, k0 / kNb);
-    if (valid) {
-        const int p = pos;
-        sh.rowmap[p] = r;
-        gmap[p] = r;
-        dev::StaticFor<0, kNb>([&](auto jc) {
-            constexpr int j = decltype(jc)::value;
-            if (j < w) {
-                LU[(k0 + p) + static_cast<size_t>(k0 + j) * n] = a[j];
-            }
-        });
-        if (p < kNb) {
-            dev::StaticFor<0, kNb / 4>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                *reinterpret_cast<float4*>(&sh.l11[p][4 * c]) =
-                    make_float4(a[4 * c], a[4 * c + 1], a[4 * c + 2], a[4 * c + 3]);
-            });
-        } else if constexpr (P == 0) {
-            dev::StaticFor<0, kNb / 4>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                *reinterpret_cast<float4*>(&sh.op.l[p - kNb][4 * c]) =
-                    make_float4(-a[4 * c], -a[4 * c + 1], -a[4 * c + 2], -a[4 * c + 3]);
-            });
-        } else {
-            dev::StaticFor<0, kNb>([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
-                uint16_t h[P];
-                SplitPlanes<P>(-a[j], h);
-                dev::StaticFor<0, P>([&](auto pc) {
-                    constexpr int q = decltype(pc)::value;
-                    sh.op.l[q][p - kNb][j] = h[q];
-                });
-            });
-        }
-    }
-    __syncthreads();
-    return 0;
-}
-
-// inv(L11) by wave 0, one column per lane, by column substitution; rows and
-// columns past the panel's width w are the identity's.  It goes to LDS (for
-// U12) and to the workspace (for the solves' diagonal blocks).
-template <int P>
-__device__ __forceinline__ void PanelInverses(const MixedSystem& s, int k0, int p, FactorShared<P>& sh)
-{
-    const int lane = dev::LaneId();
-    const int wave = static_cast<int>(threadIdx.x) / kWave;
-    const int w = min(kNb, s.n - k0);
-    float* g = s.dinv + static_cast<size_t>(p) * kInvFloats;
-    if (wave == 0 && lane < kNb) {
-        const int j = lane;
-        float x[kNb];
-        dev::StaticFor<0, kNb>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            float v = q == j ? 1.0f : 0.0f;
-            if (q < w) {
-                dev::StaticFor<0, q>([&](auto ic) {
-                    constexpr int i = decltype(ic)::value;
-                    v = fmaf(-sh.l11[q][i], x[i], v);
-                });
-            }
-            x[q] = v;
-        });
-        dev::StaticFor<0, kNb>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            sh.linv[q][j] = x[q];
-            g[q + kNb * j] = x[q];
-        });
-    }
-}
-
-// One 16-column tile of the trailing matrix: gather, U12 = inv(L11) A12,
-// update, store.  Accumulator t, element j of a lane holds row 16t + (lane & 15)
-// (panel position) and column c0 + 4 (lane >> 4) + j.
-template <int P>
-__device__ __forceinline__ void SweepTile(const MixedSystem& s, int k0, int c0, FactorShared<P>& sh)
-{
-    const int n = s.n;
-    const int lane = dev::LaneId();
-    const int wave = static_cast<int>(threadIdx.x) / kWave;
-    const int rows = n - k0;
-    const int r16 = lane & 15;
-    const int grp = lane >> 4;
-    const int cq = grp * 4;
-    float* LU = s.LU;
-    const int tiles = (rows + 15) / 16;
-
-    f32x4 acc[kMaxTiles];
-    dev::StaticFor<0, kMaxTiles>([&](auto tc) {
-        constexpr int t = decltype(tc)::value;
-        if (t < tiles) {
-            const int row = 16 * t + r16;
-            const bool ok = row < rows;
-            const size_t src = static_cast<size_t>(k0 + (ok ? sh.rowmap[row] : 0));
-            dev::StaticFor<0, 4>([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
-                const int col = c0 + cq + j;
-                acc[t][j] = (ok && col < n) ? LU[src + static_cast<size_t>(col) * n] : 0.0f;
-            });
-        }
-    });
-
-    // U12 = inv(L11) A12 (top 32 rows) on the FP32 matrix instruction, as
-    // U12^T = A12^T inv(L11)^T: A12 through the per-wave LDS tile
-    float (*ut)[kTileN + 1] = sh.ut[wave];
-    dev::StaticFor<0, 2>([&](auto tc) {
-        constexpr int t = decltype(tc)::value;
-        dev::StaticFor<0, 4>([&](auto jc) {
-            constexpr int j = decltype(jc)::value;
-            ut[16 * t + r16][cq + j] = acc[t][j];
-        });
-    });
-    WaveSync();
-    dev::StaticFor<0, 2>([&](auto bc) {
-        constexpr int b = decltype(bc)::value;
-        f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
-        dev::StaticFor<0, kNb / 4>([&](auto sc) {
-            constexpr int st = decltype(sc)::value;
-            v = __builtin_amdgcn_mfma_f32_16x16x4f32(ut[4 * st + grp][r16], sh.linv[16 * b + r16][4 * st + grp], v,
-                                                     0, 0, 0);
-        });
-        acc[b] = v;
-    });
-
-    // U12 in the update's operand format (U12^T: [col][k])
-    dev::StaticFor<0, 2>([&](auto bc) {
-        constexpr int b = decltype(bc)::value;
-        dev::StaticFor<0, 4>([&](auto jc) {
-            constexpr int j = decltype(jc)::value;
-            const int col = cq + j;
-            const int k = 16 * b + r16;
-            if constexpr (P == 0) {
-                sh.op.u[wave][col][k] = acc[b][j];
-            } else {
-                uint16_t h[P];
-                SplitPlanes<P>(acc[b][j], h);
-                dev::StaticFor<0, P>([&](auto pc) {
-                    constexpr int q = decltype(pc)::value;
-                    sh.op.u[wave][q][col][k] = h[q];
-                });
-            }
-        });
-    });
-    WaveSync();
-
-    // A22 += (-L21) U12, as acc^T += U12^T (-L21)^T, small terms first
-    if constexpr (P == 0) {
-        float ua[kNb / 4];
-        dev::StaticFor<0, kNb / 4>([&](auto sc) {
-            constexpr int st = decltype(sc)::value;
-            ua[st] = sh.op.u[wave][r16][4 * st + grp];
-        });
-        dev::StaticFor<2, kMaxTiles>([&](auto tc) {
-            constexpr int t = decltype(tc)::value;
-            if (t < tiles) {
-                const int lr = 16 * t - kNb + r16;
-                f32x4 v = acc[t];
-                dev::StaticFor<0, kNb / 4>([&](auto sc) {
-                    constexpr int st = decltype(sc)::value;
-                    v = __builtin_amdgcn_mfma_f32_16x16x4f32(ua[st], sh.op.l[lr][4 * st + grp], v, 0, 0, 0);
-                });
-                acc[t] = v;
-            }
-        });
-    } else {
-        const int kq = grp * 8;
-        bf16x8 ub[P];
-        dev::StaticFor<0, P>([&](auto pc) {
-            constexpr int q = decltype(pc)::value;
-            ub[q] = *reinterpret_cast<const bf16x8*>(&sh.op.u[wave][q][r16][kq]);
-        });
-        dev::StaticFor<2, kMaxTiles>([&](auto tc) {
-            constexpr int t = decltype(tc)::value;
-            if (t < tiles) {
-                const int lr = 16 * t - kNb + r16;
-                bf16x8 la[P];
-                dev::StaticFor<0, P>([&](auto pc) {
-                    constexpr int q = decltype(pc)::value;
-                    la[q] = *reinterpret_cast<const bf16x8*>(&sh.op.l[q][lr][kq]);
-                });
-                f32x4 v = acc[t];
-                if constexpr (P == 3) {
-                    v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ub[0], la[2], v, 0, 0, 0);
-                    v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ub[2], la[0], v, 0, 0, 0);
-                    v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ub[1], la[1], v, 0, 0, 0);
-                }
-                if constexpr (P >= 2) {
-                    v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ub[0], la[1], v, 0, 0, 0);
-                    v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ub[1], la[0], v, 0, 0, 0);
-                }
-                v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ub[0], la[0], v, 0, 0, 0);
-                acc[t] = v;
-            }
-        });
-    }
-
-    dev::StaticFor<0, kMaxTiles>([&](auto tc) {
-        constexpr int t = decltype(tc)::value;
-        if (t < tiles) {
-            const int row = 16 * t + r16;
-            if (row < rows) {
-                dev::StaticFor<0, 4>([&](auto jc) {
-                    constexpr int j = decltype(jc)::value;
-                    const int col = c0 + cq + j;
-                    if (col < n) {
-                        LU[(k0 + row) + static_cast<size_t>(col) * n] = acc[t][j];
-                    }
-                });
-            }
-        }
-    });
-    WaveSync();
-}
-
-// onlyMarked: factor only the systems the refinement marked for FP32 updates.
 template <int P>
 __global__ __launch_bounds__(kThreads, 2) void GesvFactorKernel(const MixedSystem* __restrict__ systems, int count,
                                                             int onlyMarked)
@@ -489,15 +111,15 @@ __global__ __launch_bounds__(kThreads, 2) void GesvFactorKernel(const MixedSyste
             continue;
         }
         const int n = s.n;
-        // working copy of A
+        // working copy of A: 32 columns in flight per thread
         if (t < n) {
-            for (int j0 = 0; j0 < n; j0 += 8) {
-                float v[8];
-                dev::StaticFor<0, 8>([&](auto jc) {
+            for (int j0 = 0; j0 < n; j0 += 32) {
+                float v[32];
+                dev::StaticFor<0, 32>([&](auto jc) {
                     constexpr int j = decltype(jc)::value;
                     v[j] = j0 + j < n ? s.A[t + static_cast<size_t>(j0 + j) * s.lda] : 0.0f;
                 });
-                dev::StaticFor<0, 8>([&](auto jc) {
+                dev::StaticFor<0, 32>([&](auto jc) {
                     constexpr int j = decltype(jc)::value;
                     if (j0 + j < n) {
                         s.LU[t + static_cast<size_t>(j0 + j) * n] = v[j];
@@ -509,19 +131,21 @@ __global__ __launch_bounds__(kThreads, 2) void GesvFactorKernel(const MixedSyste
             failed = 0;
         }
         __syncthreads();
+        const PanelView view{s.LU, n, n};
         for (int k0 = 0, p = 0; k0 < n; k0 += kNb, ++p) {
-            const int z = FactorPanel<P>(s, k0, sh);
+            int wl = 0;
+            const int z = lub::FactorPanel<P, true, false>(view, k0, s.rowmap + RowMapOffset(n, p), sh, &wl);
             if (z != 0) {
                 if (t == 0) {
                     failed = z;
                 }
                 break;
             }
-            PanelInverses<P>(s, k0, p, sh);
+            lub::PanelInverses<P>(view, k0, wl, s.dinv + static_cast<size_t>(p) * kInvFloats, sh);
             __syncthreads();
             const int first = k0 + kNb;  // first trailing column (panels before the last are full)
-            for (int c0 = first + wave * kTileN; c0 < n; c0 += kWaves * kTileN) {
-                SweepTile<P>(s, k0, c0, sh);
+            for (int c0 = first + wave * lub::kTileN; c0 < n; c0 += kWaves * lub::kTileN) {
+                lub::SweepTile<P>(view, k0, c0, sh);
             }
             __syncthreads();
         }
@@ -726,15 +350,22 @@ __device__ __forceinline__ double Residual(const MixedSystem& s, RefineShared& s
         r[c] = (t < n && c < nrhs) ? static_cast<double>(s.B[t + static_cast<size_t>(c) * s.ldb]) : 0.0;
     });
     if (t < n) {
-        for (int j0 = 0; j0 < n; j0 += 4) {
-            float av[4];
-            dev::StaticFor<0, 4>([&](auto jc) {
+        // eight columns of A per step, the next eight loaded while these are used
+        constexpr int kCols = 8;
+        float cur[kCols], nxt[kCols];
+        dev::StaticFor<0, kCols>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            cur[j] = j < n ? s.A[t + static_cast<size_t>(j) * s.lda] : 0.0f;
+        });
+        for (int j0 = 0; j0 < n; j0 += kCols) {
+            dev::StaticFor<0, kCols>([&](auto jc) {
                 constexpr int j = decltype(jc)::value;
-                av[j] = j0 + j < n ? s.A[t + static_cast<size_t>(j0 + j) * s.lda] : 0.0f;
+                const int col = j0 + kCols + j;
+                nxt[j] = col < n ? s.A[t + static_cast<size_t>(col) * s.lda] : 0.0f;
             });
-            dev::StaticFor<0, 4>([&](auto jc) {
+            dev::StaticFor<0, kCols>([&](auto jc) {
                 constexpr int j = decltype(jc)::value;
-                const double a = static_cast<double>(av[j]);
+                const double a = static_cast<double>(cur[j]);
                 rs += fabs(a);
                 // x rows past n are zero, so the last partial group needs no check
                 dev::StaticFor<0, kMaxRhs / 2>([&](auto cc) {
@@ -745,6 +376,10 @@ __device__ __forceinline__ double Residual(const MixedSystem& s, RefineShared& s
                         r[2 * c + 1] = fma(-a, xv.y, r[2 * c + 1]);
                     }
                 });
+            });
+            dev::StaticFor<0, kCols>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                cur[j] = nxt[j];
             });
         }
     }

@@ -14,14 +14,16 @@
 //   n <= 64  : one segment of a wave per matrix, one or two rows per lane
 //              (tuned per size bucket), the pivot row broadcast through a
 //              per-segment LDS slot (no barriers).
-//   n <= 256 : one workgroup per matrix, 2-D block-cyclic ownership (RPT rows x
-//              CPT columns per thread), pivot row and multipliers exchanged
-//              through LDS, two barriers per column.
+//   n <= 256 : blocked right-looking LU, one workgroup per matrix
+//              (getrf_blocked.hip, lu_blocked.h): 32-column panels factored in
+//              registers, trailing updates on the FP32 matrix instruction.
 //   n > 256  : one workgroup per matrix working in global memory (L2), with
 //              explicit row swaps.
-// Every path performs, element by element, the same operations in the same
-// order (l = a_ik / p, a_ij = fma(-l, u_kj, a_ij)), so all three produce
-// bit-identical factors and pivots.
+// The n <= 64 and n > 256 paths perform, element by element, the same
+// operations in the same order (l = a_ik / p, a_ij = fma(-l, u_kj, a_ij)) and
+// produce bit-identical factors and pivots; the blocked path has the same
+// pivot rule and multipliers but sums the trailing update blockwise, as
+// LAPACK's blocked sgetrf does.
 #include "device_common.h"
 #include "drivers.h"
 #include "internal.h"
@@ -135,7 +137,8 @@ void LaunchGetrf(const _aclblas_handle* h, float* const* A, int n, int lda, int*
     if (n <= 64) {
         LaunchGetrfWave<PIVOT>(st, A, n, lda, ipiv, info, batch, WaveRows(h, n, batch));
     } else if (n <= 256) {
-        LaunchGetrfBlock<PIVOT>(st, A, n, lda, ipiv, info, batch);
+        const int cus = h->cu_count > 0 ? h->cu_count : 256;
+        LaunchGetrfBlocked<PIVOT>(st, A, n, lda, ipiv, info, batch, batch < 2 * cus ? batch : 2 * cus);
     } else {
         hipLaunchKernelGGL((GetrfGlobalKernel<PIVOT>), dim3(batch), dim3(kGlobalThreads), 0, st, A, n, lda, ipiv,
                            info, batch);

@@ -16,9 +16,9 @@ void LaunchGetrfWave(hipStream_t st, float* const* A, int n, int lda, int* ipiv,
 void LaunchGetrfWaveOutOfPlace(hipStream_t st, const float* const* A, int n, int lda, float* out, float** table,
                                int* ipiv, int* info, int batch, int rows);
 
-// LU, 64 < n <= 256 (getrf_block.hip)
+// LU, 64 < n <= 256, blocked (getrf_blocked.hip); grid: persistent workgroups
 template <bool PIVOT>
-void LaunchGetrfBlock(hipStream_t st, float* const* A, int n, int lda, int* ipiv, int* info, int batch);
+void LaunchGetrfBlocked(hipStream_t st, float* const* A, int n, int lda, int* ipiv, int* info, int batch, int grid);
 
 // Triangular solves, n <= 64 (getrs_small.hip); false when the shape is not covered.
 bool LaunchGetrsSmall(hipStream_t st, bool trans, int n, int nrhs, const float* const* A, int lda, const int* piv,

@@ -78,9 +78,15 @@ def test_adjoint_autograd(handle, batch, n, nrhs):
     gA = Ad.grad.cpu().double()
     gB = Bd.grad.cpu().double()
 
-    # oracle on the same factors: dB bit-exact, dA to FP64-accumulation accuracy
-    LU = A.transpose(1, 2).contiguous().numpy()
-    LUo, pivo, info = O.getrf_batched(LU)
+    # oracle on the same factors (the GPU's: for 64 < n <= 256 its LU is blocked and rounds differently from
+    # the oracle's unblocked order): dB bit-exact, dA to FP64-accumulation accuracy
+    LUd = A.transpose(1, 2).contiguous().to(dev)
+    pivd = torch.zeros((batch, n), dtype=torch.int32, device=dev)
+    infod = torch.zeros(batch, dtype=torch.int32, device=dev)
+    assert B.sgetrf_batched(handle.value, n, B.ptr_array(LUd).data_ptr(), n, pivd.data_ptr(), infod.data_ptr(),
+                            batch) == 0
+    torch.cuda.synchronize()
+    LUo, pivo = LUd.cpu().numpy(), pivd.cpu().numpy()
     Xo = O.getrs_batched(0, LUo, pivo, Bm.transpose(1, 2).contiguous().numpy())
     dBo, dAo = O.getrs_adjoint(LUo, pivo, Xo, dX.transpose(1, 2).contiguous().numpy())
     assert np.array_equal(gB.float().transpose(1, 2).numpy(), dBo), "dB differs from the oracle's transposed solve"

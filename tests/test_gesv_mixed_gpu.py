@@ -172,3 +172,32 @@ def test_launch_events_bracket_the_kernels(handle):
         assert evs[0].elapsed_time(evs[3]) >= f + s - 1e-3
     finally:
         assert B.set_launch_events(handle.value, None, 0) == 0
+
+
+@pytest.mark.parametrize("update", ["FP32", "BF16X2"])
+def test_mixed_solve_exact_ties(handle, update):
+    """Tie-heavy integer systems A = P L U (L unit lower with one +-1 below the diagonal per row, U integer
+    upper, P a row permutation) with an integer solution: the pivot candidates of most columns tie in |a|,
+    every multiplier is 0 or +-1 and every intermediate of the factorisation and the solves a small integer
+    (exact in FP32, and in two BF16 planes), so the returned X must be the integer solution exactly - a wrong
+    tie choice or row map would break it."""
+    rng = np.random.default_rng(71)
+    systems = []
+    for n in (20, 40, 64, 100, 129, 200, 256):
+        L = np.eye(n)
+        for i in range(1, n):
+            L[i, rng.integers(0, i)] = rng.choice([-1.0, 1.0])
+        U = np.triu(rng.integers(-4, 5, size=(n, n)).astype(np.float64))
+        U[np.arange(n), np.arange(n)] = rng.integers(1, 5, size=n) * rng.choice([-1, 1], size=n)
+        A = (L @ U)[rng.permutation(n)]
+        r = 3
+        X = rng.integers(-3, 4, size=(n, r)).astype(np.float64)
+        Bm = A @ X
+        systems.append({"n": n, "nrhs": r, "cond": float(np.linalg.cond(A)),
+                        "A": np.ascontiguousarray(A.T, dtype=np.float32).reshape(-1),
+                        "B": np.ascontiguousarray(Bm.T, dtype=np.float32).reshape(-1),
+                        "X": np.ascontiguousarray(X.T, dtype=np.float32).reshape(-1)})
+    xs, iters, berr, info, fb = _run(handle, systems, 1e-6, 10, update)
+    assert (info == 0).all() and not fb.any()
+    for s, x in zip(systems, xs):
+        assert np.array_equal(x, s["X"]), (s["n"], np.abs(x - s["X"]).max())

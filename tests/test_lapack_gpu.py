@@ -4,8 +4,13 @@ The GPU library is called through its C ABI (ctypes), on the same seeded inputs
 as the oracle.  The bar: pivots and info bit-exact; LU factors, solutions and
 inverses bit-exact against the oracle (the kernels perform the same roundings in
 the same order), and within the reference's MIXED_TOLERANCE (test/frame/
-verify.h:276-333) of LAPACK, the reference's own golden.  Case lists are the
-reference's CSV tables (tests/golden/cases).
+verify.h:276-333) of LAPACK, the reference's own golden.  For 64 < n <= 256 the
+LU is blocked (LAPACK sgetrf's algorithm, trailing updates summed blockwise):
+there pivots and info are bit-exact on tie-free inputs and the factors are held
+to MIXED_TOLERANCE against the oracle and LAPACK; on tie-heavy inputs whose LU
+is exact in any summation order (integer matrices P L U, see
+test_getrf_exact_ties_all_paths) everything is bit-exact on every path.  Case
+lists are the reference's CSV tables (tests/golden/cases).
 """
 import ctypes
 
@@ -37,6 +42,21 @@ def _matrix_kind(row):
     return row["matrix_type"]
 
 
+def _blocked(n):
+    """Orders factored by the blocked kernel."""
+    return 64 < n <= 256
+
+
+def _check_factors(LU, gLU, n, rows):
+    """Bit-exact against the oracle, or MIXED_TOLERANCE for the blocked orders."""
+    if not _blocked(n):
+        assert np.array_equal(LU[rows][:, :, :n], gLU[rows][:, :, :n])
+        return
+    for b in np.nonzero(rows)[0]:
+        ok, detail = O.mixed_tolerance(LU[b, :, :n], gLU[b, :, :n])
+        assert ok, "matrix %d: %s" % (b, detail)
+
+
 # ------------------------------------------------------------------ getrf CSV
 GETRF_CASES = [r for r in load_cases("sgetrf_batched.csv")]
 
@@ -58,8 +78,8 @@ def test_getrf_reference_cases(handle, row):
     if pivot:
         assert np.array_equal(ipiv, gpiv)
     ok_rows = info == 0
-    # factors: bit-exact against the oracle
-    assert np.array_equal(LU[ok_rows][:, :, :n], gLU[ok_rows][:, :, :n])
+    # factors: bit-exact against the oracle (MIXED_TOLERANCE for the blocked orders)
+    _check_factors(LU, gLU, n, ok_rows)
     # padding rows (lda > n) untouched
     assert np.array_equal(LU[:, :, n:], A[:, :, n:])
     # and the reference's own criterion against LAPACK (skipping singular batches)
@@ -120,7 +140,76 @@ def test_getrf_sizes_pivoting(handle, n):
     gLU, gpiv, ginfo = O.getrf_batched(A.copy())
     assert np.array_equal(info, ginfo)
     assert np.array_equal(ipiv, gpiv)
+    _check_factors(LU, gLU, n, np.ones(batch, bool))
+
+
+def _exact_ties_batch(n, batch, seed):
+    """A = P L U with L unit lower holding one +-1 below the diagonal in each row, U upper with small integer
+    entries (diagonal +-1..+-4), P a random row permutation.  Partial pivoting then meets exact ties in |a| on
+    most columns (the lowest current position must win) and every multiplier stays in {0, +-1}: all
+    intermediates are small integers, exact in FP32 in any summation order (checked against FP64 on the host
+    when the test was written: identical factors and pivots, thousands of tie-driven interchanges)."""
+    rng = np.random.default_rng(seed)
+    A = np.zeros((batch, n, n), np.float32)
+    for b in range(batch):
+        L = np.eye(n)
+        for i in range(1, n):
+            L[i, rng.integers(0, i)] = rng.choice([-1.0, 1.0])
+        U = np.triu(rng.integers(-4, 5, size=(n, n)).astype(np.float64))
+        U[np.arange(n), np.arange(n)] = rng.integers(1, 5, size=n) * rng.choice([-1, 1], size=n)
+        M = (L @ U)[rng.permutation(n)]
+        A[b] = M.T.astype(np.float32)               # A[b, j, i] = M[i, j]
+    return A
+
+
+@pytest.mark.parametrize("n", [7, 32, 64, 65, 100, 128, 200, 256, 300])
+def test_getrf_exact_ties_all_paths(handle, n):
+    """Tie-heavy inputs on every kernel path (n <= 64 wave kernel, 64 < n <= 256 blocked, n > 256 global):
+    factors, pivots and info bit-exact against the oracle."""
+    batch = 4 if n <= 256 else 2
+    A = _exact_ties_batch(n, batch, 500 + n)
+    LU, ipiv, info, st = _batched_getrf(handle, A)
+    assert st == 0
+    gLU, gpiv, ginfo = O.getrf_batched(A.copy())
+    assert np.array_equal(info, ginfo)
+    assert np.array_equal(ipiv, gpiv)
     assert np.array_equal(LU, gLU)
+
+
+@pytest.mark.parametrize("n,zero_col", [(100, 70), (129, 0), (200, 191), (256, 255)])
+def test_getrf_blocked_zero_pivot(handle, n, zero_col):
+    """An exact zero pivot inside a panel of the blocked path: info = column + 1, the pivots up to that column
+    as the oracle's, later pivots untouched, and the matrix left in the oracle's stopping state (to FP32
+    accuracy)."""
+    import torch
+    batch = 3
+    A = O.normal((batch, n, n), 900 + n, n=n, batch=batch, shift=2.0)
+    A[1, zero_col, :] = 0.0                          # column zero_col of matrix 1 is zero
+    dA = to_dev(A)
+    piv = torch.full((batch, n), -7, dtype=torch.int32, device="cuda")
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    pa = B.ptr_array(dA)
+    assert B.sgetrf_batched(handle.value, n, pa.data_ptr(), n, piv.data_ptr(), info.data_ptr(), batch) == 0
+    LU, p, ih = to_host(dA), to_host(piv), to_host(info)
+    gLU, gpiv, ginfo = O.getrf_batched(A.copy())
+    assert np.array_equal(ih, ginfo) and ginfo[1] == zero_col + 1
+    assert np.array_equal(p[1, :zero_col + 1], gpiv[1, :zero_col + 1])
+    assert np.all(p[1, zero_col + 1:] == -7)
+    for b in range(batch):
+        ok, detail = O.mixed_tolerance(LU[b], gLU[b])
+        assert ok, "matrix %d: %s" % (b, detail)
+
+
+def test_getrf_blocked_no_pivot(handle):
+    """Without a pivot array the blocked path factors in place without interchanges (diagonally dominant input)."""
+    n, batch = 150, 4
+    A = O.normal((batch, n, n), 77, n=n, batch=batch, shift=float(n))
+    LU, _, info, st = _batched_getrf(handle, A, pivot=False)
+    assert st == 0 and not info.any()
+    gLU, _, ginfo = O.getrf_batched(A.copy(), pivot=False)
+    for b in range(batch):
+        ok, detail = O.mixed_tolerance(LU[b], gLU[b])
+        assert ok, detail
 
 
 def test_getrf_zero_pivot_leaves_later_pivots(handle):
