@@ -176,13 +176,14 @@ class MixedWorkload:
         r = np.array(self.rs, dtype=np.float64)
         self.lu_flop = float((2.0 * n ** 3 / 3.0).sum())
         # factor kernel, per system: A read and the factors written (4 n^2 bytes each) plus the pivot order
-        # (one int per row and panel) and the 32 x 32 diagonal-block inverses the solves use (8 KiB per panel)
+        # (one int per row and panel) and the 32 x 32 inverse of each panel's unit lower diagonal block that the
+        # solves use (4 KiB per panel)
         panels = np.ceil(n / 32.0)
         rowmap = np.array([sum(int(m) - k0 for k0 in range(0, int(m), 32)) for m in n], dtype=np.float64)
-        self.factor_bytes = float((8.0 * n * n + 4.0 * rowmap + 8192.0 * panels).sum())
+        self.factor_bytes = float((8.0 * n * n + 4.0 * rowmap + 4096.0 * panels).sum())
         # solve kernel (one solve and one FP64 residual, the pass every system makes): factors, inverses and A read,
         # B read twice, X written
-        self.refine_bytes = float((8.0 * n * n + 8192.0 * panels + 4.0 * rowmap + 12.0 * n * r).sum())
+        self.refine_bytes = float((8.0 * n * n + 4096.0 * panels + 4.0 * rowmap + 12.0 * n * r).sum())
 
     def call(self, update, events=None):
         B = self.B

@@ -14,7 +14,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libops_blas.so")
+# ACLBLAS_LIBRARY selects another build of the same ABI (the host-sanitizer build in tests)
+LIB_PATH = os.environ.get("ACLBLAS_LIBRARY") or os.path.join(_HERE, "libops_blas.so")
 
 # aclblasStatus_t (include/cann_ops_blas_common.h)
 STATUS = {

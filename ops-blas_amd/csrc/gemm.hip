@@ -194,9 +194,9 @@ aclblasStatus_t RunGemm(const _aclblas_handle* h, int ka, int kb, int ok, bool t
 }
 
 // ---------------------------------------------------------------- SgemmEx
-aclblasStatus_t ValidateSgemmEx(aclblasHandle_t handle, aclblasOperation_t ta, aclblasOperation_t tb, int m, int n,
-                                int k, int lda, int ldb, int ldc, const float* alpha, const float* beta,
-                                aclblasGemmAlgo_t algo, const float* A, const float* B, const float* C)
+aclblasStatus_t ValidateSgemmEx(aclblasHandle_t handle, const aclblasOperation_t& ta, const aclblasOperation_t& tb,
+                                int m, int n, int k, int lda, int ldb, int ldc, const float* alpha, const float* beta,
+                                const aclblasGemmAlgo_t& algo, const float* A, const float* B, const float* C)
 {
     if (handle == nullptr) {
         return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
@@ -213,7 +213,7 @@ aclblasStatus_t ValidateSgemmEx(aclblasHandle_t handle, aclblasOperation_t ta, a
     if (alpha == nullptr || beta == nullptr) {
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
-    if (algo != ACLBLAS_GEMM_DEFAULT) {
+    if (EnumBits(algo) != ACLBLAS_GEMM_DEFAULT) {
         return ACLBLAS_STATUS_NOT_SUPPORTED;
     }
     if (k > 0 && (A == nullptr || B == nullptr)) {
@@ -228,7 +228,7 @@ aclblasStatus_t ValidateSgemmEx(aclblasHandle_t handle, aclblasOperation_t ta, a
 // ---------------------------------------------------------------- GemmEx
 // Accepted (A, B, C, compute) combinations: the reference's table
 // (gemm_ex_host.cpp:45-69) plus BF16 x BF16 -> FP32 and FP16 x FP16 -> FP32.
-bool ValidDtypes(aclDataType at, aclDataType bt, aclDataType ct, aclblasComputeType_t ctype)
+bool ValidDtypes(int at, int bt, int ct, int ctype)
 {
     const bool fp8a = at == ACL_FLOAT8_E4M3FN || at == ACL_FLOAT8_E5M2;
     const bool fp8b = bt == ACL_FLOAT8_E4M3FN || bt == ACL_FLOAT8_E5M2;
@@ -246,10 +246,11 @@ bool ValidDtypes(aclDataType at, aclDataType bt, aclDataType ct, aclblasComputeT
     }
 }
 
-aclblasStatus_t ValidateGemmEx(aclblasHandle_t handle, aclblasOperation_t ta, aclblasOperation_t tb, int m, int n,
-                               int k, int lda, int ldb, int ldc, const void* alpha, const void* beta,
-                               aclDataType at, aclDataType bt, aclDataType ct, aclblasComputeType_t ctype,
-                               aclblasGemmAlgo_t algo, const void* A, const void* B, const void* C)
+aclblasStatus_t ValidateGemmEx(aclblasHandle_t handle, const aclblasOperation_t& ta, const aclblasOperation_t& tb,
+                               int m, int n, int k, int lda, int ldb, int ldc, const void* alpha, const void* beta,
+                               const aclDataType& at, const aclDataType& bt, const aclDataType& ct,
+                               const aclblasComputeType_t& ctype, const aclblasGemmAlgo_t& algo, const void* A,
+                               const void* B, const void* C)
 {
     if (handle == nullptr) {
         return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
@@ -266,10 +267,10 @@ aclblasStatus_t ValidateGemmEx(aclblasHandle_t handle, aclblasOperation_t ta, ac
     if (alpha == nullptr || beta == nullptr) {
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
-    if (algo != ACLBLAS_GEMM_DEFAULT) {
+    if (EnumBits(algo) != ACLBLAS_GEMM_DEFAULT) {
         return ACLBLAS_STATUS_NOT_SUPPORTED;
     }
-    if (!ValidDtypes(at, bt, ct, ctype)) {
+    if (!ValidDtypes(EnumBits(at), EnumBits(bt), EnumBits(ct), EnumBits(ctype))) {
         return ACLBLAS_STATUS_NOT_SUPPORTED;
     }
     if (m == 0 || n == 0) {
@@ -311,10 +312,13 @@ aclblasStatus_t GemmLaunch(const _aclblas_handle* h, aclDataType at, aclDataType
 
 using namespace aclblas;
 
-extern "C" aclblasStatus_t aclblasSgemmEx(
-    aclblasHandle_t handle, aclblasOperation_t transA, aclblasOperation_t transB, int m, int n, int k,
-    const float* alpha, const float* A, int lda, const float* B, int ldb, const float* beta, float* C, int ldc,
-    aclblasGemmAlgo_t algo)
+namespace {
+
+// aclblasSgemmEx and aclblasSgemm; enum arguments by reference (unvalidated).
+aclblasStatus_t SgemmExImpl(aclblasHandle_t handle, const aclblasOperation_t& transA,
+                            const aclblasOperation_t& transB, int m, int n, int k, const float* alpha, const float* A,
+                            int lda, const float* B, int ldb, const float* beta, float* C, int ldc,
+                            const aclblasGemmAlgo_t& algo)
 {
     // reference sgemm_ex_host.cpp:415-417: quick return before any check
     if (m == 0 || n == 0) {
@@ -341,12 +345,22 @@ extern "C" aclblasStatus_t aclblasSgemmEx(
                       transB != ACLBLAS_OP_N, m, n, k, a, A, lda, B, ldb, b, C, ldc, nullptr, nullptr);
 }
 
+}  // namespace
+
+extern "C" aclblasStatus_t aclblasSgemmEx(
+    aclblasHandle_t handle, aclblasOperation_t transA, aclblasOperation_t transB, int m, int n, int k,
+    const float* alpha, const float* A, int lda, const float* B, int ldb, const float* beta, float* C, int ldc,
+    aclblasGemmAlgo_t algo)
+{
+    return SgemmExImpl(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, algo);
+}
+
 extern "C" aclblasStatus_t aclblasSgemm(
     aclblasHandle_t handle, aclblasOperation_t transA, aclblasOperation_t transB, int m, int n, int k,
     const float* alpha, const float* A, int lda, const float* B, int ldb, const float* beta, float* C, int ldc)
 {
-    return aclblasSgemmEx(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
-                          ACLBLAS_GEMM_DEFAULT);
+    const aclblasGemmAlgo_t algo = ACLBLAS_GEMM_DEFAULT;
+    return SgemmExImpl(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, algo);
 }
 
 extern "C" aclblasStatus_t aclblasGemmEx(
@@ -380,7 +394,7 @@ namespace aclblas {
 namespace {
 
 // reference blas/gemm_batched_ex/arch35/gemm_batched_ex_host.cpp:87-118
-bool ValidBatchedDtypes(aclDataType at, aclDataType bt, aclDataType ct, aclblasComputeType_t ctype)
+bool ValidBatchedDtypes(int at, int bt, int ct, int ctype)
 {
     const bool fp8a = at == ACL_FLOAT8_E4M3FN || at == ACL_FLOAT8_E5M2;
     const bool fp8b = bt == ACL_FLOAT8_E4M3FN || bt == ACL_FLOAT8_E5M2;
@@ -399,11 +413,11 @@ bool ValidBatchedDtypes(aclDataType at, aclDataType bt, aclDataType ct, aclblasC
 // reference gemm_batched_ex_host.cpp:155-244 (order: handle, alpha, beta,
 // dimensions and batch count, leading dimensions, algo, dtypes, FP8 compute
 // type, pointer arrays when batchCount > 0)
-aclblasStatus_t ValidateGemmBatchedEx(aclblasHandle_t handle, aclblasOperation_t ta, aclblasOperation_t tb, int m,
-                                      int n, int k, int lda, int ldb, int ldc, int batch, const void* alpha,
-                                      const void* beta, aclDataType at, aclDataType bt, aclDataType ct,
-                                      aclblasComputeType_t ctype, aclblasGemmAlgo_t algo, const void* A,
-                                      const void* B, const void* C)
+aclblasStatus_t ValidateGemmBatchedEx(aclblasHandle_t handle, const aclblasOperation_t& ta,
+                                      const aclblasOperation_t& tb, int m, int n, int k, int lda, int ldb, int ldc,
+                                      int batch, const void* alpha, const void* beta, const aclDataType& at,
+                                      const aclDataType& bt, const aclDataType& ct, const aclblasComputeType_t& ctype,
+                                      const aclblasGemmAlgo_t& algo, const void* A, const void* B, const void* C)
 {
     if (handle == nullptr) {
         return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
@@ -420,10 +434,10 @@ aclblasStatus_t ValidateGemmBatchedEx(aclblasHandle_t handle, aclblasOperation_t
         ldc < MaxInt(1, m)) {
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
-    if (algo != ACLBLAS_GEMM_DEFAULT) {
+    if (EnumBits(algo) != ACLBLAS_GEMM_DEFAULT) {
         return ACLBLAS_STATUS_NOT_SUPPORTED;
     }
-    if (!ValidBatchedDtypes(at, bt, ct, ctype)) {
+    if (!ValidBatchedDtypes(EnumBits(at), EnumBits(bt), EnumBits(ct), EnumBits(ctype))) {
         return ACLBLAS_STATUS_NOT_SUPPORTED;
     }
     if (batch > 0 && (A == nullptr || B == nullptr || C == nullptr)) {
@@ -433,9 +447,10 @@ aclblasStatus_t ValidateGemmBatchedEx(aclblasHandle_t handle, aclblasOperation_t
 }
 
 // reference blas/gemm_batched/arch35/gemm_batched_host.cpp:53-97
-aclblasStatus_t ValidateSgemmBatched(aclblasHandle_t handle, aclblasOperation_t ta, aclblasOperation_t tb, int m,
-                                     int n, int k, int lda, int ldb, int ldc, int batch, const float* alpha,
-                                     const float* beta, const void* A, const void* B, const void* C)
+aclblasStatus_t ValidateSgemmBatched(aclblasHandle_t handle, const aclblasOperation_t& ta,
+                                     const aclblasOperation_t& tb, int m, int n, int k, int lda, int ldb, int ldc,
+                                     int batch, const float* alpha, const float* beta, const void* A, const void* B,
+                                     const void* C)
 {
     if (handle == nullptr) {
         return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;

@@ -36,7 +36,11 @@ using gemm::GroupedProblem;
 constexpr int kDefaultTile = 128;
 constexpr int kDefaultBlocksPerCu = 2;
 
-bool ValidTrans(aclblasOperation_t t) { return t == ACLBLAS_OP_N || t == ACLBLAS_OP_T; }
+bool ValidTrans(const aclblasOperation_t& t)
+{
+    const int v = EnumBits(t);
+    return v == ACLBLAS_OP_N || v == ACLBLAS_OP_T;
+}
 
 aclblasStatus_t Validate(aclblasHandle_t handle, int groupCount, const aclblasOperation_t* ta,
                          const aclblasOperation_t* tb, const int* m, const int* n, const int* k, const float* alpha,
@@ -219,10 +223,10 @@ float HostHalfToFloat(uint16_t h)
     return out;
 }
 
-bool IsFp8Type(aclDataType t) { return t == ACL_FLOAT8_E4M3FN || t == ACL_FLOAT8_E5M2; }
+bool IsFp8Type(int t) { return t == ACL_FLOAT8_E4M3FN || t == ACL_FLOAT8_E5M2; }
 
 // reference gemm_grouped_batched_ex_host.cpp:59-72
-bool GroupedExTypesSupported(aclDataType at, aclDataType bt, aclDataType ct, aclblasComputeType_t ctype)
+bool GroupedExTypesSupported(int at, int bt, int ct, int ctype)
 {
     if (at == ACL_FLOAT16 && bt == ACL_FLOAT16 && ct == ACL_FLOAT16) {
         return ctype == ACLBLAS_COMPUTE_16F || ctype == ACLBLAS_COMPUTE_32F;
@@ -305,7 +309,7 @@ extern "C" aclblasStatus_t aclblasGemmGroupedBatchedEx(
     if (handle == nullptr) {
         return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
     }
-    if (!GroupedExTypesSupported(Atype, Btype, Ctype, computeType)) {
+    if (!GroupedExTypesSupported(EnumBits(Atype), EnumBits(Btype), EnumBits(Ctype), EnumBits(computeType))) {
         LogError("aclblasGemmGroupedBatchedEx", "unsupported type combination A=%d B=%d C=%d compute=%d",
                  static_cast<int>(Atype), static_cast<int>(Btype), static_cast<int>(Ctype),
                  static_cast<int>(computeType));

@@ -203,14 +203,14 @@ void LaunchQuantize(hipStream_t st, dim3 grid, const float* x, long long count, 
                        scale);
 }
 
-int ScaledKind(aclDataType t)
+int ScaledKind(int t)
 {
     if (t == ACL_FLOAT8_E4M3FN) return kE4M3S;
     if (t == ACL_FLOAT8_E5M2) return kE5M2S;
     return -1;
 }
 
-int OutKindScaled(aclDataType t)
+int OutKindScaled(int t)
 {
     if (t == ACL_FLOAT) return kOutF32;
     if (t == ACL_BF16) return kOutBF16;
@@ -251,9 +251,9 @@ extern "C" aclblasStatus_t aclblasGemmExScaled(
     if (lda < MaxInt(1, ta ? k : m) || ldb < MaxInt(1, tb ? n : k) || ldc < MaxInt(1, m)) {
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
-    const int ka = ScaledKind(Atype);
-    const int kb = ScaledKind(Btype);
-    const int ok = OutKindScaled(Ctype);
+    const int ka = ScaledKind(EnumBits(Atype));
+    const int kb = ScaledKind(EnumBits(Btype));
+    const int ok = OutKindScaled(EnumBits(Ctype));
     if (ka < 0 || kb < 0 || ok < 0) {
         return ACLBLAS_STATUS_NOT_SUPPORTED;
     }

@@ -627,7 +627,7 @@ extern "C" aclblasStatus_t aclblasSgesvBatchedMixedEx(
     if (handle == nullptr) {
         return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
     }
-    const int mode = static_cast<int>(update);
+    const int mode = EnumBits(update);
     aclblasStatus_t st = ValidateMixed(batchCount, nArray, nrhsArray, Aarray, ldaArray, Barray, ldbArray, Xarray,
                                        ldxArray, tol, maxIter, mode);
     if (st != ACLBLAS_STATUS_SUCCESS) {

@@ -363,7 +363,7 @@ void SetInfo(int* info, int v)
 }
 
 // Argument checks in the reference's order (sgetrs_batched_host.cpp:49-95).
-aclblasStatus_t ValidateGetrs(aclblasOperation_t trans, int n, int nrhs, const float* const* A, int lda,
+aclblasStatus_t ValidateGetrs(const aclblasOperation_t& trans, int n, int nrhs, const float* const* A, int lda,
                               float* const* B, int ldb, int batch, int* info)
 {
     if (!IsValidOp(trans)) {

@@ -13,6 +13,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 
 #include "cann_ops_blas.h"
 
@@ -115,9 +116,23 @@ inline aclblasStatus_t LaunchStatus(const char* api)
     return ACLBLAS_STATUS_SUCCESS;
 }
 
-inline bool IsValidOp(aclblasOperation_t op)
+// C callers pass enumeration arguments as plain ints.  Reading a value outside
+// an enumeration's range through the C++ enum type is undefined behaviour, so
+// the validators take enum arguments by reference and read their bits as an
+// int; code after validation uses the enum types normally.
+template <typename E>
+inline int EnumBits(const E& e)
 {
-    return op == ACLBLAS_OP_N || op == ACLBLAS_OP_T || op == ACLBLAS_OP_C;
+    static_assert(sizeof(E) == sizeof(int), "C enumerations are int-sized");
+    int v;
+    std::memcpy(&v, &e, sizeof v);
+    return v;
+}
+
+inline bool IsValidOp(const aclblasOperation_t& op)
+{
+    const int v = EnumBits(op);
+    return v == ACLBLAS_OP_N || v == ACLBLAS_OP_T || v == ACLBLAS_OP_C;
 }
 
 inline int MaxInt(int a, int b) { return a > b ? a : b; }

@@ -164,14 +164,14 @@ extern "C" aclblasStatus_t aclblasStrsm(aclblasHandle_t handle, aclblasSideMode_
     if (handle == nullptr) {
         return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
     }
-    if (side != ACLBLAS_SIDE_LEFT && side != ACLBLAS_SIDE_RIGHT) {
+    if (EnumBits(side) != ACLBLAS_SIDE_LEFT && EnumBits(side) != ACLBLAS_SIDE_RIGHT) {
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
     if (m < 0 || n < 0) {
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
-    if ((uplo != ACLBLAS_UPPER && uplo != ACLBLAS_LOWER) || !IsValidOp(trans) ||
-        (diag != ACLBLAS_NON_UNIT && diag != ACLBLAS_UNIT) || alpha == nullptr) {
+    if ((EnumBits(uplo) != ACLBLAS_UPPER && EnumBits(uplo) != ACLBLAS_LOWER) || !IsValidOp(trans) ||
+        (EnumBits(diag) != ACLBLAS_NON_UNIT && EnumBits(diag) != ACLBLAS_UNIT) || alpha == nullptr) {
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
     if (m == 0 || n == 0) {

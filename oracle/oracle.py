@@ -15,7 +15,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+# ORACLE_LIBRARY selects another build of oracle.cpp (the sanitizer build in tests)
+_LIB_PATH = os.environ.get("ORACLE_LIBRARY") or os.path.join(_HERE, "_build", "liboracle.so")
 _lib = None
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")

@@ -343,6 +343,26 @@ def test_gemm_batched_ex_argument_rows(cpu_handle, row):
     assert st == exp
 
 
+SGEMM_BATCHED_ROWS = [r for r in load_cases("sgemm_batched.csv")
+                      if status_code(r["expect_result"]) != 0 or _gemm_degenerate(r) or int(r["batchCount"]) == 0]
+
+
+@pytest.mark.parametrize("row", SGEMM_BATCHED_ROWS, ids=[r["case_name"] for r in SGEMM_BATCHED_ROWS])
+def test_sgemm_batched_argument_rows(cpu_handle, row):
+    """Error and no-work rows of test/gemm_batched/sgemm_batched/arch35/sgemm_batched_test.csv."""
+    exp = status_code(row["expect_result"])
+    h = None if exp == status_code("HANDLE_IS_NULLPTR") else cpu_handle
+    alpha = None if _bool(row["alpha_null"]) else float(row["alpha"])
+    beta = None if _bool(row["beta_null"]) else float(row["beta"])
+    A = None if _bool(row["aarray_null"]) else DUMMY
+    Bp = None if _bool(row["barray_null"]) else DUMMY
+    C = None if _bool(row["carray_null"]) else DUMMY
+    lda, ldb, ldc = gemm_ex_lds(row)
+    st = B.sgemm_batched(h, _op(row["transA"]), _op(row["transB"]), int(row["m"]), int(row["n"]), int(row["k"]),
+                         alpha, A, lda, Bp, ldb, beta, C, ldc, int(row["batchCount"]))
+    assert st == exp
+
+
 def test_sgemm_batched_argument_checks(cpu_handle):
     ok = status_code("SUCCESS")
     inv = status_code("INVALID_VALUE")
