@@ -336,8 +336,10 @@ aclblasStatus_t aclblasGetTuningChoice(aclblasHandle_t handle, int op, int bucke
 /* Bucket of a problem: op 0 takes cls (0 FP32, 1 BF16/FP16, 2 FP8 under
  * GemmEx, 3 FP8 under GemmExScaled), units = 128 x 128 output tiles over the
  * call and k; op 1 takes units = 128 x 128 tiles over all problems; op 2 takes
- * units = number of systems; op 3 takes cls = the order n (1..64) and
- * units = the batch count.  cls and k are ignored for ops 1 and 2, k for op 3. */
+ * cls = the trailing-update class (0: BF16 and BF16X2, 1: FP32 and BF16X3) and
+ * units = number of systems; op 3 takes cls = the order n (1..64) and units =
+ * the batch count.
+ * cls and k are ignored for op 1, k for ops 2 and 3. */
 aclblasStatus_t aclblasTuningBucket(aclblasHandle_t handle, int op, int cls, long long units, int k, int* bucket);
 
 #ifdef __cplusplus

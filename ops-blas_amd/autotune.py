@@ -18,8 +18,8 @@ Buckets and representative problems:
   grouped GEMM, 3 buckets: FP32 problems with m, n, k uniform in steps of 16
     (16 problems up to 512, 40 up to 768, 64 up to 2048); candidates tile
     64/128 x 1..4 workgroups per CU.
-  mixed solve, 4 buckets: 128, 512, 2048, 10000 systems, n 16..256, nrhs
-    1..16, cond 1e2..1e6; candidates 1..2 factor x 1..4 solve workgroups per CU.
+  mixed solve, 8 buckets: BF16 or FP32 updates x 128, 512, 2048, 10000 systems, n 16..256, nrhs
+    1..16, cond 1e2..1e6; candidates 1..3 factor x 1..4 solve workgroups per CU.
   batched LU (n <= 64), 16 buckets: n = 16, 32, 64 with batches giving 1, 3,
     6 and 12 one-row-per-lane waves per SIMD (n <= 8 has one choice);
     candidates 1 or 2 matrix rows per lane.
@@ -207,11 +207,12 @@ class Tuner:
         dev = self.dev
         h = self.h.value
         counts = [128, 512, 2048, 2500 if self.quick else 10000]
-        for q, count in enumerate(counts):
+        # class 0: BF16 trailing updates (the default two planes), class 1: FP32 updates
+        for cls, q, count in [(c, q, n) for c in (0, 1) for q, n in enumerate(counts)]:
             systems = W.conditioned_systems(count, seed=100 + q)
-            bucket = self.bucket(B.TUNE_MIXED, 0, count)
-            if bucket != q:
-                raise RuntimeError("mixed representative %d: bucket %d" % (q, bucket))
+            bucket = self.bucket(B.TUNE_MIXED, cls, count)
+            if bucket != 4 * cls + q:
+                raise RuntimeError("mixed representative %d, class %d: bucket %d" % (q, cls, bucket))
             offA, offB = [], []
             oa = ob = 0
             for s in systems:
@@ -230,13 +231,15 @@ class Tuner:
             px = B.host_ptrs([dX.data_ptr() + 4 * o for o in offB])
             iters = torch.empty(count, dtype=torch.int32, device=dev)
 
+            update = B.MIXED_UPDATE["BF16X2" if cls == 0 else "FP32"]
+
             def run():
-                st = B.sgesv_batched_mixed(h, ns, rs, pa, ns, pb, ns, px, ns, 1e-6, 10, iters.data_ptr(), None,
-                                           None)
+                st = B.sgesv_batched_mixed_ex(h, ns, rs, pa, ns, pb, ns, px, ns, 1e-6, 10, update, iters.data_ptr())
                 assert st == 0, st
 
-            cands = [(f, s) for f in (1, 2) for s in (1, 2, 3, 4)]
-            self._sweep(B.TUNE_MIXED, bucket, cands, run, lambda: [dX, iters], "%d mixed systems" % count)
+            cands = [(f, s) for f in (1, 2, 3) for s in (1, 2, 3, 4)]
+            self._sweep(B.TUNE_MIXED, bucket, cands, run, lambda: [dX, iters],
+                        "%d mixed systems, %s updates" % (count, "BF16x2" if cls == 0 else "FP32"))
 
     # ------------------------------------------------------------ batched LU
     def tune_lu(self):

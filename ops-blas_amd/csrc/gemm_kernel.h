@@ -905,6 +905,14 @@ __global__ __launch_bounds__(kThreads) void GroupedGemmKernel(const GroupedProbl
 namespace aclblas {
 namespace gemm {
 
+// The direct-to-LDS kernels copy an index-contiguous operand (A with transA ==
+// N: m rows; B with transB != N: n rows) in chunks of `chunk` elements: they
+// need its extent to be a multiple of the chunk.
+inline bool DmaExtentsWhole(const GemmArgs& p, int chunk)
+{
+    return (p.transA != 0 || p.m % chunk == 0) && (p.transB == 0 || p.n % chunk == 0);
+}
+
 // Launch with a tuned choice: tile edge `tile` (256: 256 x 256 over 8 waves of
 // 128 x 64, 64 KiB of LDS per stage, so that two waves share each SIMD; 128 and
 // 64: 2 x 2 waves) and `group` tile rows per L2 group.  A 256 tile is never
@@ -915,8 +923,11 @@ inline void LaunchGemmTyped(hipStream_t st, GemmArgs p, int batch, int tile, int
     constexpr int EB = KindInfo<KA>::bytes;
     p.group = group;
     if constexpr (KA == kBF16 || KA == kF16) {
-        // direct-to-LDS kernel: one problem, 16-byte aligned operands
-        if (tile == 256 && p.Aarr == nullptr && p.vecA != 0 && p.vecB != 0) {
+        // direct-to-LDS kernel: one problem, 16-byte aligned operands, and an
+        // index-contiguous operand's extent a whole number of 16-byte chunks
+        // (its DMA copies whole chunks: a partial last chunk would read past
+        // the end of the operand's last column)
+        if (tile == 256 && p.Aarr == nullptr && p.vecA != 0 && p.vecB != 0 && DmaExtentsWhole(p, 8)) {
             p.tilesM = (p.m + 255) / 256;
             p.tilesN = (p.n + 255) / 256;
             const dim3 grid(p.tilesM * p.tilesN);
@@ -936,8 +947,9 @@ inline void LaunchGemmTyped(hipStream_t st, GemmArgs p, int batch, int tile, int
         }
     }
     if constexpr (KA == kE4M3S || KA == kE5M2S) {
-        // direct-to-LDS FP8 kernel: one problem, 16-byte aligned operands
-        if (tile == 256 && p.Aarr == nullptr && p.vecA != 0 && p.vecB != 0) {
+        // direct-to-LDS FP8 kernel: one problem, 16-byte aligned operands, whole
+        // 16-byte chunks along an index-contiguous operand
+        if (tile == 256 && p.Aarr == nullptr && p.vecA != 0 && p.vecB != 0 && DmaExtentsWhole(p, 16)) {
             p.tilesM = (p.m + 255) / 256;
             p.tilesN = (p.n + 255) / 256;
             const dim3 grid(p.tilesM * p.tilesN);

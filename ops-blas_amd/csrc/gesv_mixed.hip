@@ -98,41 +98,24 @@ struct MixedSystem {
 __device__ __forceinline__ int RowMapOffset(int n, int p) { return p * n - 16 * p * (p - 1); }
 
 template <int P>
-__global__ __launch_bounds__(kThreads, 2) void GesvFactorKernel(const MixedSystem* __restrict__ systems, int count,
+__global__ __launch_bounds__(kThreads, 3) void GesvFactorKernel(const MixedSystem* __restrict__ systems, int count,
                                                             int onlyMarked)
 {
     __shared__ __attribute__((aligned(16))) FactorShared<P> sh;
     __shared__ int failed;
     const int t = threadIdx.x;
-    const int wave = t / kWave;
     for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
         const MixedSystem s = systems[idx];
         if (onlyMarked != 0 && s.state[1] == 0) {
             continue;
         }
         const int n = s.n;
-        // working copy of A: 32 columns in flight per thread
-        if (t < n) {
-            for (int j0 = 0; j0 < n; j0 += 32) {
-                float v[32];
-                dev::StaticFor<0, 32>([&](auto jc) {
-                    constexpr int j = decltype(jc)::value;
-                    v[j] = j0 + j < n ? s.A[t + static_cast<size_t>(j0 + j) * s.lda] : 0.0f;
-                });
-                dev::StaticFor<0, 32>([&](auto jc) {
-                    constexpr int j = decltype(jc)::value;
-                    if (j0 + j < n) {
-                        s.LU[t + static_cast<size_t>(j0 + j) * n] = v[j];
-                    }
-                });
-            }
-        }
         if (t == 0) {
             failed = 0;
         }
-        __syncthreads();
-        const PanelView view{s.LU, n, n};
+        // the first panel and its sweep read A and write the working copy
         for (int k0 = 0, p = 0; k0 < n; k0 += kNb, ++p) {
+            const PanelView view = k0 == 0 ? PanelView{s.LU, n, n, s.A, s.lda} : PanelView{s.LU, n, n, s.LU, n};
             int wl = 0;
             const int z = lub::FactorPanel<P, true, false>(view, k0, s.rowmap + RowMapOffset(n, p), sh, &wl);
             if (z != 0) {
@@ -143,10 +126,7 @@ __global__ __launch_bounds__(kThreads, 2) void GesvFactorKernel(const MixedSyste
             }
             lub::PanelInverses<P>(view, k0, wl, s.dinv + static_cast<size_t>(p) * kInvFloats, sh);
             __syncthreads();
-            const int first = k0 + kNb;  // first trailing column (panels before the last are full)
-            for (int c0 = first + wave * lub::kTileN; c0 < n; c0 += kWaves * lub::kTileN) {
-                lub::SweepTile<P>(view, k0, c0, sh);
-            }
+            lub::SweepTrailing<P>(view, k0, sh);  // panels before the last are full
             __syncthreads();
         }
         if (t == 0) {
@@ -691,9 +671,13 @@ extern "C" aclblasStatus_t aclblasSgesvBatchedMixedEx(
         return st;
     }
     const int cus = handle->cu_count > 0 ? handle->cu_count : 256;
-    // persistent grids, workgroups per CU from the tuning table (built-in: two
-    // factor and three solve workgroups, what LDS and registers let one CU hold)
-    const tune::Choice c = tune::Get(handle, tune::kOpMixed, tune::MixedBucket(batchCount, cus), tune::Choice{2, 3});
+    // persistent grids, workgroups per CU from the tuning table, by the update's
+    // class: 0 for the one- and two-plane BF16 updates (their factor kernel's
+    // LDS lets three workgroups share a CU), 1 for FP32 and three planes (two);
+    // built-in: three or two factor workgroups, three solve workgroups
+    const int cls = (mode == ACLBLAS_MIXED_UPDATE_FP32 || mode == ACLBLAS_MIXED_UPDATE_BF16X3) ? 1 : 0;
+    const tune::Choice c = tune::Get(handle, tune::kOpMixed, tune::MixedBucket(cls, batchCount, cus),
+                                     tune::Choice{cls == 0 ? 3 : 2, 3});
     const unsigned gridF = static_cast<unsigned>(std::min(batchCount, c.a * cus));
     const unsigned gridS = static_cast<unsigned>(std::min(batchCount, c.b * cus));
     const MixedSystem* sys = static_cast<const MixedSystem*>(dev);

@@ -66,9 +66,9 @@ __global__ __launch_bounds__(kThreads, 2) void GetrfBlockedKernel(float* const* 
 {
     __shared__ __attribute__((aligned(16))) lub::FactorShared<0> sh;
     const int t = threadIdx.x;
-    const int wave = t / dev::kWave;
     for (int b = blockIdx.x; b < batch; b += gridDim.x) {
-        const lub::PanelView view{Aarray[b], lda, n};
+        float* const Ab = Aarray[b];
+        const lub::PanelView view{Ab, lda, n, Ab, lda};
         int infoVal = 0;
         for (int k0 = 0; k0 < n; k0 += kNb) {
             int wl = 0;
@@ -81,9 +81,7 @@ __global__ __launch_bounds__(kThreads, 2) void GetrfBlockedKernel(float* const* 
             }
             lub::PanelInverses<0>(view, k0, wl, nullptr, sh);
             __syncthreads();
-            for (int c0 = k0 + kNb + wave * lub::kTileN; c0 < n; c0 += kWaves * lub::kTileN) {
-                lub::SweepTile<0>(view, k0, c0, sh);
-            }
+            lub::SweepTrailing<0>(view, k0, sh);
             if (PIVOT) {
                 PermuteLeft(view.LU, lda, n, k0, sh.rowmap);
             }

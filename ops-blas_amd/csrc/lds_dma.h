@@ -27,8 +27,14 @@ __device__ __forceinline__ uint32_t LdsAddr(const void* p)
 // per lane) to LDS bytes [lds + 16 l, lds + 16 l + 16); lds is wave-uniform.
 // The LDS base goes through M0, which needs one wait state between its write
 // and the load that reads it.
-// (M0 is listed as clobbered; the compiler warns that it reserves M0, and no
-// code around these loads uses it.)
+// M0 is listed as clobbered; the compiler warns that it reserves M0.  In the
+// kernels that use this load no other instruction reads M0 (no dynamic
+// register indexing, no compiler-issued LDS DMA): in their gfx950 code every
+// M0 reference is this function's own s_mov_b32.  The compiler's own form,
+// __builtin_amdgcn_global_load_lds(g, lds, 16, 0, 0), is not used because the
+// compiler then waits for every outstanding LDS DMA after each barrier of the
+// main loop (an s_waitcnt vmcnt(0) after the s_barrier), which drains the
+// ring of stages in flight on every stage.
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Winline-asm"
 __device__ __forceinline__ void LdsDma16(const void* g, uint32_t lds)

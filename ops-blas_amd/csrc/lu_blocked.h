@@ -35,10 +35,16 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 // The matrix being factored: order n, column-major with leading dimension ld.
+// Loads read the current values from src (leading dimension srcLd), stores go
+// to LU: src is LU itself except on a first panel that reads the caller's
+// matrix and writes the working copy (the panel and its sweep together write
+// every element, so no separate copy pass is needed).
 struct PanelView {
     float* LU;
     int ld;
     int n;
+    const float* src;
+    int srcLd;
 };
 
 __device__ __forceinline__ uint16_t Bf16Rne(float f)
@@ -83,15 +89,27 @@ struct UpdateOperands<0> {  // FP32
 template <int P>
 struct FactorShared {
     alignas(16) float cand[2][kWaves][kNb];               // each wave's best row, by column parity
-    uint32_t candKey[2][kWaves];
-    int candPos[2][kWaves];
+    alignas(16) uint64_t cand64[2][kWaves];               // its packed key (PackCand), 0: no candidate
     int rowmap[kMaxN];                                    // panel row map (position -> row, relative)
     int pw[kNb];                                          // pivot position chosen at each panel column
-    alignas(16) float l11[kNb][kLd];                      // panel rows 0..31: L11 below, U11 on/above the diagonal
     alignas(16) float linv[kNb][kLd];                     // inv(L11), row-major
-    alignas(16) float ut[kWaves][kNb][kTileN + 1];        // per-wave top rows of the sweep tile
+    // L11 lives from the panel's write-back to its inverse, the sweep's
+    // per-wave tiles only during the sweep: they share the space
+    union {
+        alignas(16) float l11[kNb][kLd];                  // panel rows 0..31: L11 below, U11 on/above the diagonal
+        alignas(16) float ut[kWaves][kNb][kTileN + 1];    // per-wave top rows of the sweep tile
+    };
     UpdateOperands<P> op;
 };
+
+// A wave's best candidate as one integer whose maximum over the waves is the
+// pivot: |a| key, then the lowest position (0xFFFF - position), then the wave.
+// A candidate always packs to more than 0 (position < 0xFFFF).
+__device__ __forceinline__ uint64_t PackCand(uint32_t key, int pos, int wave)
+{
+    return (static_cast<uint64_t>(key) << 32) | (static_cast<uint64_t>(0xFFFF - pos) << 8) |
+           static_cast<uint64_t>(wave);
+}
 
 __device__ __forceinline__ void WaveSync()
 {
@@ -101,114 +119,138 @@ __device__ __forceinline__ void WaveSync()
 }
 
 // Panel factorisation by the whole workgroup: wave v holds rows 64v..64v+63 of
-// the panel (one row per lane) in registers.  Per column, each wave reduces
-// its candidates with DPP, the wave's best lane publishes its row, |a| key and
-// position in LDS (double-buffered by column parity), one barrier, and every
-// lane picks the same winner among the four (without PIVOT the row at
-// position j is the only candidate).  Multipliers are a / p (DIVIDE) or
-// a * (1 / p).  The panel goes back in pivoted row order with its row map
-// (sh.rowmap, and gmap when given), L11 / U11 in sh.l11 and -L21 in the
-// update's operand format; sh.pw[j] is the position the pivot of column j
-// came from.  At an exact zero pivot in column k0 + j the panel stops there:
-// *wl = j (columns factored), the columns right of it hold the state after
-// the first j steps, and the -L21 operand columns from j on are zero, so a
-// sweep with this panel applies exactly those j steps to the trailing
-// columns.  Returns the zero-pivot column + 1, or 0 (*wl = width then).
-// Uniform over the workgroup; ends with a barrier.
+// the panel (one row per lane) in registers.  Per column, each wave finds its
+// best candidate (largest |a| key by a DPP reduction; the lanes holding it by
+// ballot, and only when several do, the lowest position among them by a
+// second reduction), that lane publishes its row, key and position in LDS
+// (double-buffered by column parity), one barrier, and every lane reads the
+// four waves' keys and positions as two vectors and picks the same winner
+// (without PIVOT the row at position j is the only candidate).  Waves with no
+// rows left in the panel only take part in the barriers.  Multipliers are
+// a / p (DIVIDE) or a * (1 / p).  The panel goes back
+// in pivoted row order with its row map (sh.rowmap, and gmap when given),
+// L11 / U11 in sh.l11 and -L21 in the update's operand format; sh.pw[j] is the
+// position the pivot of column j came from.  At an exact zero pivot in column
+// k0 + j the panel stops there: *wl = j (columns factored), the columns right
+// of it hold the state after the first j steps, and the -L21 operand columns
+// from j on are zero, so a sweep with this panel applies exactly those j steps
+// to the trailing columns.  Returns the zero-pivot column + 1, or 0 (*wl =
+// width then).  Uniform over the workgroup; ends with a barrier.
 template <int P, bool PIVOT, bool DIVIDE>
 __device__ __forceinline__ int FactorPanel(const PanelView& pv, int k0, int* gmap, FactorShared<P>& sh, int* wl)
 {
     const int n = pv.n;
     const size_t ld = static_cast<size_t>(pv.ld);
+    const size_t sld = static_cast<size_t>(pv.srcLd);
     const int lane = dev::LaneId();
-    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    const int wave = dev::Uniform(static_cast<int>(threadIdx.x) / kWave);
     const int rows = n - k0;
     const int w = min(kNb, rows);
     const int r = wave * kWave + lane;
     const bool valid = r < rows;
+    const bool active = wave * kWave < rows;  // the wave holds panel rows
     float* LU = pv.LU;
 
     float a[kNb];
     dev::StaticFor<0, kNb>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
-        a[j] = (valid && j < w) ? LU[(k0 + r) + static_cast<size_t>(k0 + j) * ld] : 0.0f;
+        a[j] = (valid && j < w) ? pv.src[(k0 + r) + static_cast<size_t>(k0 + j) * sld] : 0.0f;
     });
+    if (!active && lane == 0) {
+        sh.cand64[0][wave] = 0;
+        sh.cand64[1][wave] = 0;
+    }
     int pos = r;
     int zeroCol = 0;
 
+    // Straight-line per column (no early exit, so the row stays in the same
+    // registers): a column past the panel's width or after a zero pivot is
+    // skipped as a whole by every wave alike.
     dev::StaticFor<0, kNb>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
-        if (j >= w || zeroCol != 0) {
-            return;
-        }
         constexpr int buf = j & 1;
-        const bool cand = valid && (PIVOT ? pos >= j : pos == j);
-        const uint32_t key = cand ? dev::MagKey(a[j]) : 0u;
-        // wave best: largest key, then lowest position (non-candidates lose)
-        const uint32_t best = dev::SegReduce<kWave, true>(key);
-        const bool tied = cand && key == best;
-        const uint32_t lowest = dev::SegReduce<kWave, false>(tied ? static_cast<uint32_t>(pos) : 0xFFFFu);
-        const bool mine = tied && static_cast<uint32_t>(pos) == lowest;
-        if (mine) {
-            dev::StaticFor<0, kNb / 4>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                *reinterpret_cast<float4*>(&sh.cand[buf][wave][4 * c]) =
-                    make_float4(a[4 * c], a[4 * c + 1], a[4 * c + 2], a[4 * c + 3]);
-            });
-            sh.candKey[buf][wave] = key;
-            sh.candPos[buf][wave] = pos;
-        } else if (lane == 0 && lowest == 0xFFFFu) {
-            sh.candPos[buf][wave] = 0xFFFF;  // no candidate in this wave
-            sh.candKey[buf][wave] = 0;
-        }
-        __syncthreads();
-        int wb = -1;
-        uint32_t bk = 0;
-        int bp = 0xFFFF;
-        dev::StaticFor<0, kWaves>([&](auto vc) {
-            constexpr int v = decltype(vc)::value;
-            const int p = sh.candPos[buf][v];
-            const uint32_t k = sh.candKey[buf][v];
-            if (p != 0xFFFF && (wb < 0 || k > bk || (k == bk && p < bp))) {
-                wb = v;
-                bk = k;
-                bp = p;
+        const bool live = j < w && zeroCol == 0;  // uniform over the workgroup
+        if (live && active) {
+            const bool cand = valid && (PIVOT ? pos >= j : pos == j);
+            const uint32_t key = cand ? dev::MagKey(a[j]) : 0u;
+            const uint32_t best = dev::SegReduce<kWave, true>(key);
+            const bool tied = cand && key == best;
+            uint64_t tmask = __builtin_amdgcn_ballot_w64(tied);
+            if (tmask & (tmask - 1)) {
+                // several lanes hold the best key: the lowest position wins
+                const uint32_t lowest = dev::SegReduce<kWave, false>(tied ? static_cast<uint32_t>(pos) : 0xFFFFu);
+                tmask = __builtin_amdgcn_ballot_w64(tied && static_cast<uint32_t>(pos) == lowest);
             }
-        });
-        wb = dev::Uniform(wb);
-        const int pw = dev::Uniform(bp);
-        if (threadIdx.x == 0) {
-            sh.pw[j] = pw;
+            if (tmask == 0) {
+                if (lane == 0) {
+                    sh.cand64[buf][wave] = 0;  // no candidate in this wave
+                }
+            } else if (lane == static_cast<int>(__builtin_ctzll(tmask))) {
+                dev::StaticFor<0, kNb / 4>([&](auto cc) {
+                    constexpr int c = decltype(cc)::value;
+                    *reinterpret_cast<float4*>(&sh.cand[buf][wave][4 * c]) =
+                        make_float4(a[4 * c], a[4 * c + 1], a[4 * c + 2], a[4 * c + 3]);
+                });
+                sh.cand64[buf][wave] = PackCand(key, pos, wave);
+            }
         }
+        if (live) {
+            __syncthreads();
+        }
+        float pivot = 0.0f;
+        int pw = 0;
         float u[kNb];
-        dev::StaticFor<j / 4, kNb / 4>([&](auto cc) {
-            constexpr int c = decltype(cc)::value;
-            const float4 v = *reinterpret_cast<const float4*>(&sh.cand[buf][wb][4 * c]);
-            u[4 * c] = v.x;
-            u[4 * c + 1] = v.y;
-            u[4 * c + 2] = v.z;
-            u[4 * c + 3] = v.w;
-        });
-        const float pivot = u[j];
-        if (pivot == 0.0f) {
-            zeroCol = k0 + j + 1;
-            return;
-        }
-        const bool winner = pos == pw && cand;
-        if (pos == j) {
-            pos = pw;
-        }
-        if (winner) {
-            pos = j;
-        }
-        const float rcp = DIVIDE ? 0.0f : 1.0f / pivot;
-        if (valid && pos > j) {
-            const float l = DIVIDE ? a[j] / pivot : a[j] * rcp;
-            a[j] = l;
-            dev::StaticFor<j + 1, kNb>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                a[c] = fmaf(-l, u[c], a[c]);
+        if (live) {
+            // the winner over the waves: the largest packed candidate
+            uint64_t best4 = sh.cand64[buf][0];
+            dev::StaticFor<1, kWaves>([&](auto vc) {
+                constexpr int v = decltype(vc)::value;
+                const uint64_t c = sh.cand64[buf][v];
+                best4 = c > best4 ? c : best4;
             });
+            const uint32_t lo = static_cast<uint32_t>(best4);
+            const int wb = dev::Uniform(static_cast<int>(lo & 0xFFu));
+            pw = dev::Uniform(0xFFFF - static_cast<int>((lo >> 8) & 0xFFFFu));
+            if (threadIdx.x == 0) {
+                sh.pw[j] = pw;
+            }
+            dev::StaticFor<j / 4, kNb / 4>([&](auto cc) {
+                constexpr int c = decltype(cc)::value;
+                const float4 v = *reinterpret_cast<const float4*>(&sh.cand[buf][wb][4 * c]);
+                u[4 * c] = v.x;
+                u[4 * c + 1] = v.y;
+                u[4 * c + 2] = v.z;
+                u[4 * c + 3] = v.w;
+            });
+            pivot = u[j];
+            if (pivot == 0.0f) {
+                zeroCol = k0 + j + 1;
+            }
+        }
+        const bool step = live && pivot != 0.0f;  // uniform
+        const bool winner = step && valid && pos == pw && (PIVOT ? pos >= j : pos == j);
+        pos = (step && pos == j) ? pw : pos;
+        pos = winner ? j : pos;
+        if (step && active) {
+            const bool upd = valid && pos > j;
+            if constexpr (DIVIDE) {
+                // the reference's multipliers; rows left alone keep their exact values
+                const float l = a[j] / pivot;
+                a[j] = upd ? l : a[j];
+                dev::StaticFor<j + 1, kNb>([&](auto cc) {
+                    constexpr int c = decltype(cc)::value;
+                    a[c] = upd ? fmaf(-l, u[c], a[c]) : a[c];
+                });
+            } else {
+                // rows left alone get l = 0: a - 0 * u keeps their value (up to the sign of a zero, and
+                // for finite u), which is all the backward-error-checked solve needs
+                const float l = upd ? a[j] * (1.0f / pivot) : 0.0f;
+                a[j] = upd ? l : a[j];
+                dev::StaticFor<j + 1, kNb>([&](auto cc) {
+                    constexpr int c = decltype(cc)::value;
+                    a[c] = fmaf(-l, u[c], a[c]);
+                });
+            }
         }
     });
     const int done = zeroCol != 0 ? zeroCol - 1 - k0 : w;
@@ -228,10 +270,16 @@ __device__ __forceinline__ int FactorPanel(const PanelView& pv, int k0, int* gma
             }
         });
         if (p < kNb) {
+            // multiplier columns past a zero pivot are not multipliers: zero (the inverse treats them as
+            // the identity's)
             dev::StaticFor<0, kNb / 4>([&](auto cc) {
                 constexpr int c = decltype(cc)::value;
-                *reinterpret_cast<float4*>(&sh.l11[p][4 * c]) =
-                    make_float4(a[4 * c], a[4 * c + 1], a[4 * c + 2], a[4 * c + 3]);
+                float v[4];
+                dev::StaticFor<0, 4>([&](auto ec) {
+                    constexpr int e = decltype(ec)::value;
+                    v[e] = (4 * c + e < p && 4 * c + e >= done) ? 0.0f : a[4 * c + e];
+                });
+                *reinterpret_cast<float4*>(&sh.l11[p][4 * c]) = make_float4(v[0], v[1], v[2], v[3]);
             });
         } else if constexpr (P == 0) {
             dev::StaticFor<0, kNb / 4>([&](auto cc) {
@@ -258,7 +306,8 @@ __device__ __forceinline__ int FactorPanel(const PanelView& pv, int k0, int* gma
 
 // inv(L11) by wave 0, one column per lane, by column substitution, where L11
 // holds the panel's first wl multiplier columns (the rest of the unit lower
-// block is the identity's; rows and columns past the panel's width too).  It
+// block is the identity's: the panel wrote zeros there; rows and columns past
+// the panel's width are the identity's too).  It
 // goes to LDS (for U12) and, when g is given, to g (32 x 32, column-major).
 template <int P>
 __device__ __forceinline__ void PanelInverses(const PanelView& pv, int k0, int wl, float* g, FactorShared<P>& sh)
@@ -273,11 +322,17 @@ __device__ __forceinline__ void PanelInverses(const PanelView& pv, int k0, int w
             constexpr int q = decltype(qc)::value;
             float v = q == j ? 1.0f : 0.0f;
             if (q < w) {
-                dev::StaticFor<0, q>([&](auto ic) {
-                    constexpr int i = decltype(ic)::value;
-                    if (i < wl) {
-                        v = fmaf(-sh.l11[q][i], x[i], v);
-                    }
+                // row q of L11, four multipliers per read (columns past a zero pivot hold zeros)
+                dev::StaticFor<0, (q + 3) / 4>([&](auto cc) {
+                    constexpr int c = decltype(cc)::value;
+                    const float4 rv = *reinterpret_cast<const float4*>(&sh.l11[q][4 * c]);
+                    const float r4[4] = {rv.x, rv.y, rv.z, rv.w};
+                    dev::StaticFor<0, 4>([&](auto ec) {
+                        constexpr int i = 4 * decltype(cc)::value + decltype(ec)::value;
+                        if constexpr (i < q) {
+                            v = fmaf(-r4[i - 4 * c], x[i], v);
+                        }
+                    });
                 });
             }
             x[q] = v;
@@ -292,24 +347,24 @@ __device__ __forceinline__ void PanelInverses(const PanelView& pv, int k0, int w
     }
 }
 
-// One 16-column tile of the trailing matrix: gather, U12 = inv(L11) A12,
-// update, store.  Accumulator t, element j of a lane holds row 16t + (lane & 15)
-// (panel position) and column c0 + 4 (lane >> 4) + j.
+// One 16-column tile of the trailing matrix in accumulators: element j of
+// accumulator t of a lane holds row 16t + (lane & 15) (panel position) and
+// column c0 + 4 (lane >> 4) + j.
+typedef f32x4 SweepAcc[kMaxTiles];
+
+// Gathers the tile at column c0 through the panel's row map.
 template <int P>
-__device__ __forceinline__ void SweepTile(const PanelView& pv, int k0, int c0, FactorShared<P>& sh)
+__device__ __forceinline__ void GatherTile(const PanelView& pv, int k0, int c0, const FactorShared<P>& sh,
+                                           SweepAcc& acc)
 {
     const int n = pv.n;
-    const size_t ld = static_cast<size_t>(pv.ld);
+    const size_t ld = static_cast<size_t>(pv.srcLd);
     const int lane = dev::LaneId();
-    const int wave = static_cast<int>(threadIdx.x) / kWave;
     const int rows = n - k0;
     const int r16 = lane & 15;
-    const int grp = lane >> 4;
-    const int cq = grp * 4;
-    float* LU = pv.LU;
+    const int cq = (lane >> 4) * 4;
+    const float* LU = pv.src;
     const int tiles = (rows + 15) / 16;
-
-    f32x4 acc[kMaxTiles];
     dev::StaticFor<0, kMaxTiles>([&](auto tc) {
         constexpr int t = decltype(tc)::value;
         if (t < tiles) {
@@ -323,6 +378,22 @@ __device__ __forceinline__ void SweepTile(const PanelView& pv, int k0, int c0, F
             });
         }
     });
+}
+
+// The gathered tile: U12 = inv(L11) A12, update, store in position order.
+template <int P>
+__device__ __forceinline__ void UpdateTile(const PanelView& pv, int k0, int c0, FactorShared<P>& sh, SweepAcc& acc)
+{
+    const int n = pv.n;
+    const size_t ld = static_cast<size_t>(pv.ld);
+    const int lane = dev::LaneId();
+    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    const int rows = n - k0;
+    const int r16 = lane & 15;
+    const int grp = lane >> 4;
+    const int cq = grp * 4;
+    float* LU = pv.LU;
+    const int tiles = (rows + 15) / 16;
 
     // U12 = inv(L11) A12 (top 32 rows) on the FP32 matrix instruction, as
     // U12^T = A12^T inv(L11)^T: A12 through the per-wave LDS tile
@@ -434,6 +505,19 @@ __device__ __forceinline__ void SweepTile(const PanelView& pv, int k0, int c0, F
         }
     });
     WaveSync();
+}
+
+// The trailing columns k0 + width .. n-1 of the panel at k0, in 16-column
+// tiles dealt to the waves in turn.
+template <int P>
+__device__ __forceinline__ void SweepTrailing(const PanelView& pv, int k0, FactorShared<P>& sh)
+{
+    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    for (int c0 = k0 + kNb + wave * kTileN; c0 < pv.n; c0 += kWaves * kTileN) {
+        SweepAcc acc;
+        GatherTile<P>(pv, k0, c0, sh, acc);
+        UpdateTile<P>(pv, k0, c0, sh, acc);
+    }
 }
 
 // onlyMarked: factor only the systems the refinement marked for FP32 updates.

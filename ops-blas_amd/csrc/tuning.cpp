@@ -49,11 +49,11 @@ int GroupedBucket(long long tiles128, int cus)
     return SizeBucket(tiles128, cus, kEdges, kGroupedBuckets);
 }
 
-int MixedBucket(long long count, int cus)
+int MixedBucket(int cls, long long count, int cus)
 {
-    // < cus, < 4 cus, < 16 cus, more
+    // < cus, < 4 cus, < 16 cus, more; then the update class (0: BF16, 1: FP32)
     static const long long kEdges[4] = {4, 16, 64, 0};
-    return SizeBucket(count, cus, kEdges, kMixedBuckets);
+    return cls * kMixedSizeBuckets + SizeBucket(count, cus, kEdges, kMixedSizeBuckets);
 }
 
 int LuBucket(int S, long long batch, int cus)
@@ -187,7 +187,10 @@ extern "C" aclblasStatus_t aclblasTuningBucket(aclblasHandle_t handle, int op, i
             *bucket = tune::GroupedBucket(units, cus);
             return ACLBLAS_STATUS_SUCCESS;
         case tune::kOpMixed:
-            *bucket = tune::MixedBucket(units, cus);
+            if (cls < 0 || cls > 1) {
+                return ACLBLAS_STATUS_INVALID_VALUE;
+            }
+            *bucket = tune::MixedBucket(cls, units, cus);
             return ACLBLAS_STATUS_SUCCESS;
         case tune::kOpLu:
             if (cls < 1 || cls > 64) {

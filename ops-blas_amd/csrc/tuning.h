@@ -18,7 +18,8 @@ namespace tune {
 enum Op : int {
     kOpGemm = 0,     // a = tile edge (64, 128, 256), b = tile rows per L2 group (1..64)
     kOpGrouped = 1,  // a = tile edge (64, 128), b = workgroups per CU of the persistent grid (1..4)
-    kOpMixed = 2,    // a = factor workgroups per CU (1..4), b = solve workgroups per CU (1..4)
+    kOpMixed = 2,    // a = factor workgroups per CU (1..4), b = solve workgroups per CU (1..4); class 0: one-
+                     // and two-plane BF16 trailing updates, 1: FP32 and three planes
     kOpLu = 3,       // a = rows per lane of the n <= 64 LU (1, 2; 1 only for n <= 8), b ignored
     kNumOps = 4
 };
@@ -34,7 +35,8 @@ enum GemmClass : int { kClassF32 = 0, kClass16 = 1, kClassF8 = 2, kClassF8Scaled
 constexpr int kGemmSizeBuckets = 5;
 constexpr int kGemmKBuckets = 3;
 constexpr int kGroupedBuckets = 3;
-constexpr int kMixedBuckets = 4;
+constexpr int kMixedSizeBuckets = 4;
+constexpr int kMixedBuckets = 2 * kMixedSizeBuckets;  // update class x system count
 constexpr int kLuBuckets = 16;
 
 // GEMM bucket from the element class, the number of 128 x 128 output tiles
@@ -43,7 +45,7 @@ int GemmBucket(int cls, long long tiles128, int k, int cus);
 // Grouped GEMM bucket from the total number of 128 x 128 tiles of all problems.
 int GroupedBucket(long long tiles128, int cus);
 // Mixed-solve bucket from the number of systems.
-int MixedBucket(long long count, int cus);
+int MixedBucket(int cls, long long count, int cus);
 // n <= 64 LU bucket from the padded order S (8, 16, 32, 64) and the waves a
 // one-row-per-lane launch would have (batch * S / 64) relative to the SIMD
 // count (4 per CU): below 2, 4, 8 per SIMD, or more.

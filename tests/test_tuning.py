@@ -47,8 +47,10 @@ def test_gemm_bucket_boundaries(h):
 def test_grouped_and_mixed_buckets(h):
     assert [_bucket(h, B.TUNE_GROUPED, 0, u) for u in (0, CUS - 1, CUS, 4 * CUS - 1, 4 * CUS, 10 ** 9)] == \
         [0, 0, 1, 1, 2, 2]
-    assert [_bucket(h, B.TUNE_MIXED, 0, u) for u in (1, CUS - 1, CUS, 4 * CUS, 16 * CUS - 1, 16 * CUS)] == \
-        [0, 0, 1, 2, 2, 3]
+    for cls in (0, 1):
+        assert [_bucket(h, B.TUNE_MIXED, cls, u) for u in (1, CUS - 1, CUS, 4 * CUS, 16 * CUS - 1, 16 * CUS)] == \
+            [4 * cls + b for b in (0, 0, 1, 2, 2, 3)]
+    assert B.tuning_bucket(h, B.TUNE_MIXED, 2, 10)[0] == status_code("INVALID_VALUE")
 
 
 def test_lu_buckets(h):
