@@ -12,8 +12,10 @@
  *
  * aclDataType values: ACL_FLOAT=0, ACL_FLOAT16=1 and ACL_BF16=27 are the values
  * the reference states (reference test/frame/csv_loader.h:330).  The FP8 values
- * are not stated anywhere in the reference; 35 (E5M2) and 36 (E4M3FN) follow the
- * public CANN enum numbering.
+ * are not stated anywhere in the reference's library headers; 35 (E5M2) and 36
+ * (E4M3FN) follow the public CANN enum numbering, and 36 and 40 (FP4 E2M1, the
+ * BLASLt MX format) are the values the reference's BLASLt tests parse
+ * (reference test/blasLtMatmul/blasLtMatmul_param.h:21-31).
  */
 #ifndef ACLBLAS_AMD_ACL_ACL_H
 #define ACLBLAS_AMD_ACL_ACL_H
@@ -43,7 +45,8 @@ typedef enum {
     ACL_BOOL = 12,
     ACL_BF16 = 27,
     ACL_FLOAT8_E5M2 = 35,
-    ACL_FLOAT8_E4M3FN = 36
+    ACL_FLOAT8_E4M3FN = 36,
+    ACL_FLOAT4_E2M1 = 40
 } aclDataType;
 
 #ifdef __cplusplus

@@ -75,7 +75,49 @@ _SIGNATURES = {
     "aclblasSetTuningOverride": [_P, _I, _I, _I],
     "aclblasGetTuningChoice": [_P, _I, _I, _P, _P],
     "aclblasTuningBucket": [_P, _I, _I, ctypes.c_longlong, _I, _P],
+    # include/cann_ops_blasLt.h
+    "aclblasLtGetVersion": [_P],
+    "aclblasLtGetProperty": [_I, _P],
+    "aclblasLtCreate": [_P],
+    "aclblasLtDestroy": [_P],
+    "aclblasLtMatrixLayoutCreate": [_P, _I, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64],
+    "aclblasLtMatrixLayoutDestroy": [_P],
+    "aclblasLtMatrixLayoutSetAttribute": [_P, _I, _P, ctypes.c_size_t],
+    "aclblasLtMatrixLayoutGetAttribute": [_P, _I, _P, ctypes.c_size_t, _P],
+    "aclblasLtMatmulDescCreate": [_P, _I, _I],
+    "aclblasLtMatmulDescDestroy": [_P],
+    "aclblasLtMatmulDescInit": [_P, _I, _I],
+    "aclblasLtMatmulDescSetAttribute": [_P, _I, _P, ctypes.c_size_t],
+    "aclblasLtMatmulDescGetAttribute": [_P, _I, _P, ctypes.c_size_t, _P],
+    "aclblasLtMatmulPreferenceCreate": [_P],
+    "aclblasLtMatmulPreferenceDestroy": [_P],
+    "aclblasLtMatmulPreferenceSetAttribute": [_P, _I, _P, ctypes.c_size_t],
+    "aclblasLtMatmulPreferenceGetAttribute": [_P, _I, _P, ctypes.c_size_t, _P],
+    "aclblasLtMatmulAlgoInit": [_P, _I, _I, _I, _I, _I, _I, _I, _P],
+    "aclblasLtMatmulAlgoGetIds": [_P, _I, _I, _I, _I, _I, _I, _P, _I, _P],
+    "aclblasLtMatmulAlgoGetHeuristic": [_P, _P, _P, _P, _P, _P, _P, _I, _P, _P],
+    "aclblasLtMatmul": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_size_t, _P],
 }
+
+# BLASLt (include/cann_ops_blasLt.h)
+ACL_FLOAT4_E2M1 = 40
+LT_ORDER_COL, LT_ORDER_ROW = 0, 1
+LT_LAYOUT_BATCH_COUNT, LT_LAYOUT_OFFSET, LT_LAYOUT_TYPE, LT_LAYOUT_ORDER, LT_LAYOUT_ROWS, LT_LAYOUT_COLS, \
+    LT_LAYOUT_LD = range(7)
+LT_DESC_TRANSA, LT_DESC_TRANSB, LT_DESC_EPILOGUE, LT_DESC_BIAS_POINTER, LT_DESC_BIAS_DATA_TYPE, \
+    LT_DESC_A_SCALE_POINTER, LT_DESC_B_SCALE_POINTER = range(7)
+LT_DESC_A_SCALE_MODE, LT_DESC_B_SCALE_MODE = 31, 32
+LT_EPILOGUE_DEFAULT, LT_EPILOGUE_RELU = 1, 2
+LT_PREF_SEARCH_MODE, LT_PREF_MAX_WORKSPACE_BYTES = 0, 1
+
+
+class LtAlgo(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_uint8 * 16), ("max_workspace_bytes", ctypes.c_size_t)]
+
+
+class LtHeuristicResult(ctypes.Structure):
+    _fields_ = [("algo", LtAlgo), ("workspaceSize", ctypes.c_size_t), ("state", ctypes.c_int),
+                ("wavesCount", ctypes.c_float), ("reserved", ctypes.c_int * 4)]
 
 # tuning operations (aclblasSetTuningOverride)
 TUNE_GEMM, TUNE_GROUPED, TUNE_MIXED, TUNE_LU = 0, 1, 2, 3
@@ -334,3 +376,41 @@ def ptr_array(t):
     ptrs = torch.tensor([base + b * stride for b in range(batch)], dtype=torch.int64, device=t.device)
     t._aclblas_ptrs = ((base, stride, batch), ptrs)
     return ptrs
+
+
+# ---------------------------------------------------------------- BLASLt helpers
+def lt_create():
+    h = _P()
+    st = load().aclblasLtCreate(ctypes.byref(h))
+    return st, h
+
+
+def lt_layout(dtype, rows, cols, ld, order=LT_ORDER_ROW):
+    """A matrix layout with ORDER set (the reference's callers set ROW); returns (status, layout)."""
+    lay = _P()
+    st = load().aclblasLtMatrixLayoutCreate(ctypes.byref(lay), int(dtype), int(rows), int(cols), int(ld))
+    if st == 0 and order is not None:
+        o = ctypes.c_int32(order)
+        st = load().aclblasLtMatrixLayoutSetAttribute(lay, LT_LAYOUT_ORDER, ctypes.byref(o), 4)
+    return st, lay
+
+
+def lt_desc(compute=2, scale_type=ACL_FLOAT, transA=OP_N, transB=OP_N, scaleA=None, scaleB=None):
+    d = _P()
+    st = load().aclblasLtMatmulDescCreate(ctypes.byref(d), compute, scale_type)
+    for attr, v in ((LT_DESC_TRANSA, transA), (LT_DESC_TRANSB, transB)):
+        if st == 0:
+            x = ctypes.c_int32(v)
+            st = load().aclblasLtMatmulDescSetAttribute(d, attr, ctypes.byref(x), 4)
+    for attr, v in ((LT_DESC_A_SCALE_POINTER, scaleA), (LT_DESC_B_SCALE_POINTER, scaleB)):
+        if st == 0 and v is not None:
+            x = ctypes.c_void_p(v)
+            st = load().aclblasLtMatmulDescSetAttribute(d, attr, ctypes.byref(x), ctypes.sizeof(ctypes.c_void_p))
+    return st, d
+
+
+def lt_matmul(h, desc, alpha, A, Adesc, B, Bdesc, beta, C, Cdesc, D, Ddesc, algo=None, workspace=None, ws_size=0,
+              stream=None):
+    ap = ctypes.byref(algo) if algo is not None else None
+    return load().aclblasLtMatmul(h, desc, _fp(alpha), A, Adesc, B, Bdesc, _fp(beta), C, Cdesc, D, Ddesc, ap,
+                                  workspace, ws_size, stream)

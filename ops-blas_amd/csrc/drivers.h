@@ -37,4 +37,19 @@ aclblasStatus_t SgemmBatchedLaunch(const _aclblas_handle* h, bool transA, bool t
                                    float alpha, const float* const* A, int lda, const float* const* B, int ldb,
                                    float beta, float* const* C, int ldc, int batch);
 
+// C = alpha op(A) op(B) + beta C for a validated problem (any m, n, k; k == 0
+// or alpha == 0 scales C), element types as aclblasGemmEx, on the handle's
+// stream with its tuned tile choice, gemm.hip.
+aclblasStatus_t GemmLaunch(const _aclblas_handle* h, aclDataType at, aclDataType bt, aclDataType ct, bool transA,
+                           bool transB, int m, int n, int k, float alpha, const void* A, int lda, const void* B,
+                           int ldb, float beta, void* C, int ldc, const float* scaleA, const float* scaleB);
+
+// Row-major MX GEMM D = alpha op(A) op(B) + beta C (FP32 C, FP32 or BF16 D):
+// MXFP8 E4M3FN (fp4 false) or MXFP4 E2M1 operands with E8M0 block scales in
+// the layouts aclblasLtMatmul states; k a multiple of 32; lda / ldb are the
+// bytes per stored row, lt_mx.hip.
+aclblasStatus_t LtMxLaunch(hipStream_t st, bool fp4, bool transA, bool transB, int m, int n, int k, float alpha,
+                           const void* A, long long lda, const void* B, long long ldb, const void* scaleA,
+                           const void* scaleB, float beta, const float* C, int ldc, void* D, int ldd, bool bf16Out);
+
 }  // namespace aclblas

@@ -11,7 +11,8 @@
 //   - batched LU with partial pivoting, the matching triangular solves and the
 //     inverse, with the reference's pivot, zero-pivot and info rules;
 //   - GEMM with FP64 accumulation and the FP16/BF16/FP8 conversions of the
-//     reference's host helpers;
+//     reference's host helpers, and the BLASLt matmul golden (FP32, MXFP8,
+//     MXFP4 with E8M0 block scales);
 //   - an FP64 residual / backward-error evaluator (the "FP64 oracle").
 // Each function cites the reference file:line it follows.  How each is pinned
 // (golden vectors, LAPACK through scipy, closed forms) is stated in the tests.
@@ -834,6 +835,99 @@ void orc_gemm(int transA, int transB, int m, int n, int k, double alpha, const v
                 r += beta * static_cast<double>(LoadAs(C, ic, Ctype));
             }
             StoreAs(C, ic, Ctype, static_cast<float>(r));
+        }
+    });
+}
+
+// BLASLt matmul golden (reference test/blasLtMatmul/blasLtMatmul_golden.h):
+// row-major D = alpha op(A) op(B) + beta C with every operand decoded to FP32
+// first and the products summed in FP64, one rounding to FP32, then (BF16 D)
+// the golden's truncation to BF16 (:318-323: the high 16 bits).  MX decoding
+// as the golden's: E4M3 with exponent field 15 read as 0 (:25-39; test data
+// stays below it), E2M1 per the OCP table (:43-63), E8M0 scale 0 -> 0,
+// 255 -> inf, else 2^(s - 127) (:65-71), one scale per 32 K elements looked
+// up in the two layouts of :77-91 (A transposed and B not transposed use the
+// K-major pairs layout).  Atype: 0 FP32, 36 MXFP8, 40 MXFP4 (both operands
+// of one kind).  lda / ldb: elements for FP32 and FP8, bytes per row for FP4.
+// C may be null (beta ignored then).  D: m x ldd floats.
+static float LtE4m3(uint8_t v)
+{
+    const int sign = (v >> 7) & 1, e = (v >> 3) & 0xF, mant = v & 7;
+    float r;
+    if ((e == 0 && mant == 0) || e == 15) {
+        r = 0.0f;
+    } else if (e == 0) {
+        r = static_cast<float>(mant) / 8.0f * std::ldexp(1.0f, -6);
+    } else {
+        r = (1.0f + static_cast<float>(mant) / 8.0f) * std::ldexp(1.0f, e - 7);
+    }
+    return sign ? -r : r;
+}
+static float LtE2m1(uint8_t nib)
+{
+    const int sign = (nib >> 3) & 1, e = (nib >> 1) & 3, mant = nib & 1;
+    const float r = e == 0 ? 0.5f * static_cast<float>(mant) : (1.0f + 0.5f * mant) * std::ldexp(1.0f, e - 1);
+    return sign ? -r : r;
+}
+static float LtE8m0(uint8_t s)
+{
+    if (s == 0) return 0.0f;
+    if (s == 255) return INFINITY;
+    return std::ldexp(1.0f, static_cast<int>(s) - 127);
+}
+static float LtScale(const uint8_t* scale, bool kMajorPairs, int minorDim, int k, int minor, int kk)
+{
+    const int block = kk / 64, sub = (kk % 64) / 32;
+    if (kMajorPairs) {
+        return LtE8m0(scale[static_cast<size_t>(block) * minorDim * 2 + static_cast<size_t>(minor) * 2 + sub]);
+    }
+    const int stride = ((k + 63) / 64) * 2;
+    return LtE8m0(scale[static_cast<size_t>(minor) * stride + static_cast<size_t>(block) * 2 + sub]);
+}
+// op(A)[i][kk] (idx = i, K index kk) of a stored operand: element (r, c) of the stored matrix with the
+// (minor, K) roles given by kContig (stored row = minor index).
+static float LtElem(int type, const void* X, long long ld, bool kContig, int minor, int kk, const uint8_t* scale,
+                    bool kMajorPairs, int minorDim, int k)
+{
+    const long long r = kContig ? minor : kk, c = kContig ? kk : minor;
+    if (type == 0) {
+        return static_cast<const float*>(X)[r * ld + c];
+    }
+    const float s = LtScale(scale, kMajorPairs, minorDim, k, minor, kk);
+    const uint8_t* b = static_cast<const uint8_t*>(X);
+    if (type == 36) {
+        return LtE4m3(b[r * ld + c]) * s;
+    }
+    const uint8_t byte = b[r * ld + c / 2];
+    return LtE2m1(static_cast<uint8_t>((c & 1) ? (byte >> 4) : (byte & 0xF))) * s;
+}
+
+void orc_lt_matmul(int type, int bf16Out, int transA, int transB, int m, int n, int k, float alpha, const void* A,
+                   long long lda, const void* B, long long ldb, const uint8_t* scaleA, const uint8_t* scaleB,
+                   float beta, const float* C, int ldc, float* D, int ldd, int threads)
+{
+    ParallelFor(m, threads, [&](int i) {
+        for (int j = 0; j < n; ++j) {
+            double sum = 0.0;
+            for (int kk = 0; kk < k; ++kk) {
+                // A: stored m x k (N, K-contiguous, scales [m][S]) or k x m (T, K-major pairs)
+                const float a = LtElem(type, A, lda, transA == 0, i, kk, scaleA, transA != 0, m, k);
+                // B: stored k x n (N, K-major pairs) or n x k (T, K-contiguous, scales [n][S])
+                const float b = LtElem(type, B, ldb, transB != 0, j, kk, scaleB, transB == 0, n, k);
+                sum += static_cast<double>(a) * static_cast<double>(b);
+            }
+            double cv = 0.0;
+            if (C != nullptr && beta != 0.0f) {
+                cv = static_cast<double>(C[static_cast<size_t>(i) * ldc + j]) * static_cast<double>(beta);
+            }
+            float v = static_cast<float>(static_cast<double>(alpha) * sum + cv);
+            if (bf16Out) {
+                uint32_t u;
+                std::memcpy(&u, &v, 4);
+                u &= 0xFFFF0000u;
+                std::memcpy(&v, &u, 4);
+            }
+            D[static_cast<size_t>(i) * ldd + j] = v;
         }
     });
 }

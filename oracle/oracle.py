@@ -68,6 +68,9 @@ def lib():
         L.orc_gemm.argtypes = [c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_double, c.c_void_p, c.c_int,
                                c.c_int, c.c_void_p, c.c_int, c.c_int, c.c_double, c.c_void_p, c.c_int, c.c_int,
                                c.c_double, c.c_int]
+        L.orc_lt_matmul.argtypes = [c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_float,
+                                    c.c_void_p, c.c_longlong, c.c_void_p, c.c_longlong, c.c_void_p, c.c_void_p,
+                                    c.c_float, c.c_void_p, c.c_int, _f32p, c.c_int, c.c_int]
         L.orc_f32_to_bf16.argtypes = [_f32p, _u16p, c.c_size_t]
         L.orc_bf16_to_f32.argtypes = [_u16p, _f32p, c.c_size_t]
         L.orc_f32_to_f16.argtypes = [_f32p, _u16p, c.c_size_t]
@@ -224,6 +227,24 @@ def gemm(transA, transB, m, n, k, alpha, A, Atype, lda, B, Btype, ldb, beta, C, 
     lib().orc_gemm(int(transA), int(transB), m, n, k, float(alpha), _ptr(A), Atype, lda, _ptr(B), Btype, ldb,
                    float(beta), _ptr(C), Ctype, ldc, float(scale_ab), nthreads or threads())
     return C
+
+
+LT_FP32, LT_MXFP8, LT_MXFP4 = 0, 36, 40
+
+
+def lt_matmul(type_, bf16_out, transA, transB, m, n, k, alpha, A, lda, B, ldb, scaleA, scaleB, beta, C, ldc, ldd,
+              nthreads=None):
+    """Row-major BLASLt golden: returns D (m x ldd float32; BF16 results truncated to BF16, as the reference's
+    golden).  A, B: numpy arrays of the stored operands (float32 for FP32, uint8 bytes for MX); scaleA/B: uint8 E8M0
+    bytes (MX) or None; C: float32 (m x ldc) or None."""
+    D = np.zeros((m, ldd), dtype=np.float32)
+    lib().orc_lt_matmul(int(type_), int(bool(bf16_out)), int(transA), int(transB), m, n, k, float(alpha),
+                        _ptr(np.ascontiguousarray(A)), lda, _ptr(np.ascontiguousarray(B)), ldb,
+                        _ptr(None if scaleA is None else np.ascontiguousarray(scaleA, dtype=np.uint8)),
+                        _ptr(None if scaleB is None else np.ascontiguousarray(scaleB, dtype=np.uint8)),
+                        float(beta), _ptr(None if C is None else np.ascontiguousarray(C, dtype=np.float32)),
+                        ldc, D.reshape(-1), ldd, nthreads or threads())
+    return D
 
 
 def to_bf16(x):

@@ -24,6 +24,7 @@ TABLES = {
     "gemm_grouped_batched_ex.csv": "test/gemm_grouped_batched_ex/arch35/gemm_grouped_batched_ex_test.csv",
     "sgemm_batched.csv": "test/gemm_batched/sgemm_batched/arch35/sgemm_batched_test.csv",
     "strsm.csv": "test/trsm/strsm/arch35/strsm_test.csv",
+    "blaslt_matmul.csv": "test/blasLtMatmul/arch35/blasLtMatmul_test.csv",
 }
 
 

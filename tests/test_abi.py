@@ -16,11 +16,11 @@ import aclblas as B
 from common import load_cases, status_code
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "cann_ops_blas.h")
+HEADERS = [os.path.join(ROOT, "include", "cann_ops_blas.h"), os.path.join(ROOT, "include", "cann_ops_blasLt.h")]
 
 
 def _declared():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     return sorted(set(re.findall(r"aclblasStatus_t\s+(aclblas\w+)\s*\(", text)))
 
 
