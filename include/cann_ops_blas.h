@@ -316,6 +316,8 @@ aclblasStatus_t aclblasSetLaunchEvents(aclblasHandle_t handle, void* const event
  * ops-blas_amd/autotune.py.  Operations and their two parameters:
  *   0 GEMM (GemmEx, Sgemm, SgemmEx, GemmBatchedEx, GemmExScaled):
  *     a = output tile edge 64 / 128 / 256 (256: 1- and 2-byte operands only),
+ *     or for FP32 32 (32 x 32 tiles, eight k-blocks of loads in flight) and
+ *     65 (64 x 64 tiles, four k-blocks in flight),
  *     b = tile rows walked together per L2 group, 1..64;
  *   1 grouped GEMM: a = tile edge 64 / 128, b = persistent workgroups per CU 1..4;
  *   2 mixed solve: a = factor workgroups per CU 1..4, b = solve workgroups per CU 1..4;

@@ -13,7 +13,8 @@ an output sums its products, so a difference is a library bug and stops the run.
 Buckets and representative problems:
   GEMM, 4 element classes x 5 tile-count buckets x 3 k buckets: square
     m = n giving cus/8, cus/2, 2 cus, 8 cus, 32 cus tiles of 128 x 128, and
-    k = 128, 1024, 4096; candidates tile 64/128/256 (no 256 for FP32) x L2
+    k = 128, 1024, 4096; candidates tile 64/128/256 (FP32: 32/65/64/128, where
+    32 and 65 are 32 x 32 and 64 x 64 tiles with deep load prefetch) x L2
     group height 4/8/16.
   grouped GEMM, 3 buckets: FP32 problems with m, n, k uniform in steps of 16
     (16 problems up to 512, 40 up to 768, 64 up to 2048); candidates tile
@@ -113,15 +114,17 @@ class Tuner:
         self.log("op %d bucket %2d %-40s best %d/%d %.3f ms" % (op, bucket, what, best[0], best[1], best_ms))
 
     # ------------------------------------------------------------------ GEMM
-    def tune_gemm(self):
+    def tune_gemm(self, classes=None):
         torch = self.torch
         targets = [self.cus / 8, self.cus / 2, 2 * self.cus, 8 * self.cus, 32 * self.cus]
         ks = [128, 1024, 4096]
-        classes = [B.CLASS_F32, B.CLASS_16, B.CLASS_F8, B.CLASS_F8_SCALED]
+        if classes is None:
+            classes = [B.CLASS_F32, B.CLASS_16, B.CLASS_F8, B.CLASS_F8_SCALED]
         groups = [8] if self.quick else [4, 8, 16]
         rng = np.random.default_rng(0)
         for cls in classes:
-            tiles = [64, 128] if cls == B.CLASS_F32 else [64, 128, 256]
+            # FP32 32 and 65: 32 x 32 and 64 x 64 tiles with deep load prefetch
+            tiles = [32, 65, 64, 128] if cls == B.CLASS_F32 else [64, 128, 256]
             for s, tgt in enumerate(targets):
                 m = max(64, int(round(128 * math.sqrt(tgt) / 16)) * 16)
                 for kb, k in enumerate(ks):
@@ -271,13 +274,13 @@ class Tuner:
 
     # ------------------------------------------------------------------ output
     def write_table(self, path, keep_from=None):
-        """Writes the measured rows plus, from `keep_from`, the rows of operations not tuned in this run."""
+        """Writes the measured rows plus, from `keep_from`, the rows of buckets not tuned in this run."""
         props = self.torch.cuda.get_device_properties(0)
         dev_name = "%s (%s)" % (props.name, getattr(props, "gcnArchName", "").split(":")[0])
-        tuned = {op for op, _, _, _ in self.rows}
+        tuned = {(op, bucket) for op, bucket, _, _ in self.rows}
         if keep_from and os.path.exists(keep_from):
             for op, bucket, a, b in read_table(keep_from):
-                if op not in tuned:
+                if (op, bucket) not in tuned:
                     self.rows.append((op, bucket, a, b))
         with open(path, "w") as f:
             f.write("// Launch-choice table {op, bucket, a, b}: written by ops-blas_amd/autotune.py\n")
@@ -314,9 +317,11 @@ def main(argv=None):
     ap.add_argument("--table", default=os.path.join(HERE, "csrc", "tuning_table.inc"))
     ap.add_argument("--report", default=None, help="markdown summary of every measurement")
     ap.add_argument("--ops", default="gemm,grouped,mixed,lu", help="comma list of gemm, grouped, mixed, lu")
+    ap.add_argument("--gemm-classes", default="f32,16,f8,f8s",
+                    help="GEMM element classes to tune: comma list of f32, 16, f8, f8s")
     ap.add_argument("--quick", action="store_true", help="one L2 group height, fewer repetitions")
     ap.add_argument("--keep-from", default=os.path.join(HERE, "csrc", "tuning_table.inc"),
-                    help="table whose rows for operations not tuned in this run are carried over")
+                    help="table whose rows for buckets not tuned in this run are carried over")
     args = ap.parse_args(argv)
     import torch
     if not torch.cuda.is_available():
@@ -327,7 +332,8 @@ def main(argv=None):
     try:
         ops = set(args.ops.split(","))
         if "gemm" in ops:
-            tuner.tune_gemm()
+            names = {"f32": B.CLASS_F32, "16": B.CLASS_16, "f8": B.CLASS_F8, "f8s": B.CLASS_F8_SCALED}
+            tuner.tune_gemm([names[c] for c in args.gemm_classes.split(",")])
         if "grouped" in ops:
             tuner.tune_grouped()
         if "mixed" in ops:

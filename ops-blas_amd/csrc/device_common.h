@@ -107,6 +107,53 @@ __device__ __forceinline__ void StaticFor(F&& f)
     }
 }
 
+// Global memory through the global address space.  Through a generic pointer
+// the compiler issues flat instructions, which count on lgkmcnt as well as
+// vmcnt, so every later wait for an LDS access would also wait for all flat
+// loads in flight; global_load / global_store count on vmcnt only.  The
+// pointers must point to device (global) memory.
+#define ACL_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const ACL_GLOBAL T* G(const T* p)
+{
+    return (const ACL_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ ACL_GLOBAL T* G(T* p)
+{
+    return (ACL_GLOBAL T*)p;
+}
+// Whole-object loads and stores of 1, 2, 4, 8 or 16 bytes.  They move the bits
+// as a plain integer or integer vector: HIP's vector types (uint4, float4, ...)
+// are classes whose copy constructor takes a generic reference, which would
+// turn the access back into a flat one.
+template <int N>
+struct GlobalBits;
+template <>
+struct GlobalBits<1> { using T = uint8_t; };
+template <>
+struct GlobalBits<2> { using T = uint16_t; };
+template <>
+struct GlobalBits<4> { using T = uint32_t; };
+template <>
+struct GlobalBits<8> { typedef uint32_t T __attribute__((ext_vector_type(2))); };
+template <>
+struct GlobalBits<16> { typedef uint32_t T __attribute__((ext_vector_type(4))); };
+
+template <typename T>
+__device__ __forceinline__ T LoadG(const void* p)
+{
+    using B = typename GlobalBits<sizeof(T)>::T;
+    const B b = *(const ACL_GLOBAL B*)p;
+    return __builtin_bit_cast(T, b);
+}
+template <typename T>
+__device__ __forceinline__ void StoreG(void* p, const T& v)
+{
+    using B = typename GlobalBits<sizeof(T)>::T;
+    *(ACL_GLOBAL B*)p = __builtin_bit_cast(B, v);
+}
+
 // Makes a wave-uniform value provably scalar (lives in an SGPR).
 __device__ __forceinline__ int Uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 

@@ -27,6 +27,8 @@
 
 #include <cstdint>
 
+#include "device_common.h"
+
 namespace aclblas {
 namespace gemm {
 
@@ -122,15 +124,11 @@ struct TileLoader {
     static_assert(kChunks % NT == 0, "tile must split evenly");
     static_assert(kMChunks * (kElems / kPerThread) == NT, "transposed blocks cover the tile");
     static constexpr int kWords = kPerThread * EB / 4;  // 32-bit words per transposed row segment
-    static constexpr int kSegBlocks = kWords >= 4 ? kWords / 4 : 1;
     uint4 v[kPerThread];
     // addressing of the unchecked path (interior tiles), fixed for one tile
     const uint8_t* base;                 // this thread's chunk 0 at k-block 0
     size_t qstride;                      // bytes between this thread's consecutive chunks
     size_t kstep;                        // bytes per k-block
-    uint32_t soff[kPerThread];           // LDS offsets of the k-contiguous stores
-    uint32_t toff[E][kSegBlocks];        // LDS offsets of the transposed row segments
-
 
     __device__ __forceinline__ void Setup(const uint8_t* X, int ld, bool kcontig, int r0)
     {
@@ -141,50 +139,59 @@ struct TileLoader {
             base = X + (static_cast<size_t>(r0 + r) * ld + kc * E) * EB;
             qstride = static_cast<size_t>(NT / (kRowBytes / 16)) * ld * EB;
             kstep = static_cast<size_t>(kElems) * EB;
-#pragma unroll
-            for (int q = 0; q < kPerThread; ++q) {
-                const int id = t + NT * q;
-                soff[q] = SwizzledOffset(id / (kRowBytes / 16), (id % (kRowBytes / 16)) * 16);
-            }
         } else {
             const int mc = t % kMChunks;
             const int kg = t / kMChunks;
             base = X + (static_cast<size_t>(kg * kPerThread) * ld + r0 + mc * E) * EB;
             qstride = static_cast<size_t>(ld) * EB;
             kstep = static_cast<size_t>(kElems) * ld * EB;
-            const int byte0 = kg * kPerThread * EB;
-#pragma unroll
-            for (int i = 0; i < E; ++i) {
-#pragma unroll
-                for (int b = 0; b < kSegBlocks; ++b) {
-                    toff[i][b] = SwizzledOffset(mc * E + i, byte0 + b * 16);
-                }
-            }
         }
     }
 
     // All chunks of k-block kb in range and 16-byte aligned.
-    __device__ __forceinline__ void FetchFull(int kb)
+    __device__ __forceinline__ void FetchFull(int kb) { FetchFullInto(kb, v); }
+    __device__ __forceinline__ void FetchFullInto(int kb, uint4 (&dst)[kPerThread]) const
     {
         const uint8_t* p = base + static_cast<size_t>(kb) * kstep;
 #pragma unroll
         for (int q = 0; q < kPerThread; ++q) {
-            v[q] = *reinterpret_cast<const uint4*>(p + q * qstride);
+            dst[q] = dev::LoadG<uint4>(p + q * qstride);
         }
     }
 
-    // Store with the offsets of Setup (EB 2 and 4 use the register transpose).
+    // Stores the fetched chunks (EB 2 and 4 use the register transpose).  The
+    // LDS offsets are recomputed at each store: offsets kept in arrays per
+    // layout end up in private memory once the compiler merges the two
+    // layouts' stores, and a private-memory load waits for every global load
+    // in flight.
     __device__ __forceinline__ void StoreFast(uint8_t* lds, bool kcontig) const
     {
+        if constexpr (EB == 4 || EB == 2) {
+            StoreFastFrom(lds, kcontig, v);
+        } else if (kcontig) {
+            StoreFastFrom(lds, kcontig, v);
+        } else {
+            Store(lds, kcontig);
+        }
+    }
+    // The same from registers src (EB 2 and 4, or a k-contiguous operand).
+    __device__ __forceinline__ void StoreFastFrom(uint8_t* lds, bool kcontig, const uint4 (&src)[kPerThread]) const
+    {
+        const int t = threadIdx.x;
         if (kcontig) {
 #pragma unroll
             for (int q = 0; q < kPerThread; ++q) {
-                *reinterpret_cast<uint4*>(lds + soff[q]) = v[q];
+                const int id = t + NT * q;
+                *reinterpret_cast<uint4*>(lds + SwizzledOffset(id / (kRowBytes / 16), (id % (kRowBytes / 16)) * 16)) =
+                    src[q];
             }
             return;
         }
+        const int mc = t % kMChunks;
+        const int kg = t / kMChunks;
+        const int byte0 = kg * kPerThread * EB;
         if constexpr (EB == 4 || EB == 2) {
-            const uint32_t* w = reinterpret_cast<const uint32_t*>(v);
+            const uint32_t* w = reinterpret_cast<const uint32_t*>(src);
 #pragma unroll
             for (int i = 0; i < E; ++i) {
                 uint32_t seg[kWords];
@@ -198,34 +205,33 @@ struct TileLoader {
                         seg[o] = __builtin_amdgcn_perm(hi, lo, (i & 1) ? 0x07060302u : 0x05040100u);
                     }
                 }
+                const int row = mc * E + i;
                 if constexpr (kWords >= 4) {
 #pragma unroll
                     for (int o = 0; o < kWords; o += 4) {
-                        *reinterpret_cast<uint4*>(lds + toff[i][o / 4]) =
+                        *reinterpret_cast<uint4*>(lds + SwizzledOffset(row, byte0 + o * 4)) =
                             make_uint4(seg[o], seg[o + 1], seg[o + 2], seg[o + 3]);
                     }
                 } else if constexpr (kWords == 2) {
-                    *reinterpret_cast<uint2*>(lds + toff[i][0]) = make_uint2(seg[0], seg[1]);
+                    *reinterpret_cast<uint2*>(lds + SwizzledOffset(row, byte0)) = make_uint2(seg[0], seg[1]);
                 } else {
-                    *reinterpret_cast<uint32_t*>(lds + toff[i][0]) = seg[0];
+                    *reinterpret_cast<uint32_t*>(lds + SwizzledOffset(row, byte0)) = seg[0];
                 }
             }
-        } else {
-            Store(lds, kcontig);
         }
     }
 
     __device__ __forceinline__ static uint32_t LoadElem(const uint8_t* p)
     {
-        if (EB == 4) return *reinterpret_cast<const uint32_t*>(p);
-        if (EB == 2) return *reinterpret_cast<const uint16_t*>(p);
-        return *p;
+        if (EB == 4) return dev::LoadG<uint32_t>(p);
+        if (EB == 2) return dev::LoadG<uint16_t>(p);
+        return dev::LoadG<uint8_t>(p);
     }
 
     __device__ __forceinline__ static uint4 LoadChunk(const uint8_t* p, bool full, int valid)
     {
         if (full) {
-            return *reinterpret_cast<const uint4*>(p);
+            return dev::LoadG<uint4>(p);
         }
         uint32_t w[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
@@ -239,6 +245,11 @@ struct TileLoader {
 
     __device__ __forceinline__ void Fetch(const uint8_t* X, int ld, bool kcontig, bool vec, int r0, int rlim,
                                           int k0, int klim)
+    {
+        FetchInto(X, ld, kcontig, vec, r0, rlim, k0, klim, v);
+    }
+    __device__ __forceinline__ void FetchInto(const uint8_t* X, int ld, bool kcontig, bool vec, int r0, int rlim,
+                                              int k0, int klim, uint4 (&v)[kPerThread]) const
     {
         const int t = threadIdx.x;
         if (kcontig) {
@@ -508,22 +519,22 @@ enum OutKind { kOutF32 = 0, kOutBF16 = 1, kOutF16 = 2, kOutF16A = 3 };
 template <int OK>
 __device__ __forceinline__ float LoadC(const void* C, size_t idx)
 {
-    if (OK == kOutF32) return static_cast<const float*>(C)[idx];
-    if (OK == kOutBF16) return FromBf16(static_cast<const uint16_t*>(C)[idx]);
-    return FromF16(static_cast<const uint16_t*>(C)[idx]);
+    if (OK == kOutF32) return dev::G(static_cast<const float*>(C))[idx];
+    if (OK == kOutBF16) return FromBf16(dev::G(static_cast<const uint16_t*>(C))[idx]);
+    return FromF16(dev::G(static_cast<const uint16_t*>(C))[idx]);
 }
 
 template <int OK>
 __device__ __forceinline__ void StoreC(void* C, size_t idx, float v)
 {
     if (OK == kOutF32) {
-        static_cast<float*>(C)[idx] = v;
+        dev::G(static_cast<float*>(C))[idx] = v;
     } else if (OK == kOutBF16) {
-        static_cast<uint16_t*>(C)[idx] = ToBf16(v);
+        dev::G(static_cast<uint16_t*>(C))[idx] = ToBf16(v);
     } else if (OK == kOutF16A) {
-        static_cast<uint16_t*>(C)[idx] = ToF16Away(v);
+        dev::G(static_cast<uint16_t*>(C))[idx] = ToF16Away(v);
     } else {
-        static_cast<uint16_t*>(C)[idx] = ToF16(v);
+        dev::G(static_cast<uint16_t*>(C))[idx] = ToF16(v);
     }
 }
 
@@ -534,10 +545,10 @@ __device__ __forceinline__ void LoadC4(const void* C, size_t idx, bool full, int
 {
     if (full) {
         if (OK == kOutF32) {
-            const float4 x = *reinterpret_cast<const float4*>(static_cast<const float*>(C) + idx);
+            const float4 x = dev::LoadG<float4>(static_cast<const float*>(C) + idx);
             c[0] = x.x; c[1] = x.y; c[2] = x.z; c[3] = x.w;
         } else {
-            const uint2 x = *reinterpret_cast<const uint2*>(static_cast<const uint16_t*>(C) + idx);
+            const uint2 x = dev::LoadG<uint2>(static_cast<const uint16_t*>(C) + idx);
             const uint16_t h[4] = {static_cast<uint16_t>(x.x), static_cast<uint16_t>(x.x >> 16),
                                    static_cast<uint16_t>(x.y), static_cast<uint16_t>(x.y >> 16)};
 #pragma unroll
@@ -554,15 +565,16 @@ __device__ __forceinline__ void StoreC4(void* C, size_t idx, bool full, int vali
 {
     if (full) {
         if (OK == kOutF32) {
-            *reinterpret_cast<float4*>(static_cast<float*>(C) + idx) = make_float4(v[0], v[1], v[2], v[3]);
+            dev::StoreG(static_cast<float*>(C) + idx, make_float4(v[0], v[1], v[2], v[3]));
         } else {
             uint16_t h[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 h[e] = OK == kOutBF16 ? ToBf16(v[e]) : (OK == kOutF16A ? ToF16Away(v[e]) : ToF16(v[e]));
             }
-            *reinterpret_cast<uint2*>(static_cast<uint16_t*>(C) + idx) =
-                make_uint2(h[0] | (static_cast<uint32_t>(h[1]) << 16), h[2] | (static_cast<uint32_t>(h[3]) << 16));
+            dev::StoreG(static_cast<uint16_t*>(C) + idx,
+                        make_uint2(h[0] | (static_cast<uint32_t>(h[1]) << 16),
+                                   h[2] | (static_cast<uint32_t>(h[3]) << 16)));
         }
         return;
     }
@@ -647,6 +659,53 @@ __device__ __forceinline__ void StoreTileC(const TileJob& q, const f32x4 (&acc)[
     }
 }
 
+// The MFMAs of one k-block held in LDS (a, b: the stage's A and B rows; offA,
+// offB: a wave's swizzled fragment offsets).  Every tile shape runs the same
+// instruction sequence per 16 x 16 accumulator, so results do not depend on
+// the tile.
+template <int KA, int KB, int TM, int TN>
+__device__ __forceinline__ void KBlockMfma(const uint8_t* a, const uint8_t* b, const uint32_t (&offA)[4],
+                                           const uint32_t (&offB)[4], f32x4 (&acc)[TM][TN])
+{
+    if constexpr (Mfma<KA, KB>::kSplit) {
+        // half-quarter by half-quarter: half the fragment registers live
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            uint4 fb[TN];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                fb[j] = *reinterpret_cast<const uint4*>(b + ((offB[j % 4] ^ (16u * h)) + j * 16 * kRowBytes));
+            }
+#pragma unroll
+            for (int i = 0; i < TM; ++i) {
+                const uint4 fa = *reinterpret_cast<const uint4*>(a + ((offA[i % 4] ^ (16u * h)) +
+                                                                      i * 16 * kRowBytes));
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    Mfma<KA, KB>::RunHalf(fa, fb[j], acc[i][j]);
+                }
+            }
+        }
+    } else {
+        uint4 fb[TN][2];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            fb[j][0] = *reinterpret_cast<const uint4*>(b + (offB[j % 4] + j * 16 * kRowBytes));
+            fb[j][1] = *reinterpret_cast<const uint4*>(b + ((offB[j % 4] ^ 16u) + j * 16 * kRowBytes));
+        }
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+            uint4 fa[2];
+            fa[0] = *reinterpret_cast<const uint4*>(a + (offA[i % 4] + i * 16 * kRowBytes));
+            fa[1] = *reinterpret_cast<const uint4*>(a + ((offA[i % 4] ^ 16u) + i * 16 * kRowBytes));
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                Mfma<KA, KB>::Run(fa, fb[j], acc[i][j]);
+            }
+        }
+    }
+}
+
 // Computes C[m0:m0+BM, n0:n0+BN] = alpha * op(A) op(B) + beta * C for one job
 // with the LDS stages sA/sB.  FP8 non-finite results become NaN (reference
 // test/gemm/gemm_ex/gemm_ex_golden.h, QuantizeGoldenOutput: the reference's FP8
@@ -726,45 +785,7 @@ __device__ __forceinline__ void GemmTile(const TileJob& q, int m0, int n0, uint8
         if (more) {
             fetch(kb + 1);
         }
-        const uint8_t* a = sA[cur];
-        const uint8_t* b = sB[cur];
-        if constexpr (Mfma<KA, KB>::kSplit) {
-            // half-quarter by half-quarter: half the fragment registers live
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                uint4 fb[TN];
-#pragma unroll
-                for (int j = 0; j < TN; ++j) {
-                    fb[j] = *reinterpret_cast<const uint4*>(b + ((offB[j % 4] ^ (16u * h)) + j * 16 * kRowBytes));
-                }
-#pragma unroll
-                for (int i = 0; i < TM; ++i) {
-                    const uint4 fa = *reinterpret_cast<const uint4*>(a + ((offA[i % 4] ^ (16u * h)) +
-                                                                          i * 16 * kRowBytes));
-#pragma unroll
-                    for (int j = 0; j < TN; ++j) {
-                        Mfma<KA, KB>::RunHalf(fa, fb[j], acc[i][j]);
-                    }
-                }
-            }
-        } else {
-            uint4 fb[TN][2];
-#pragma unroll
-            for (int j = 0; j < TN; ++j) {
-                fb[j][0] = *reinterpret_cast<const uint4*>(b + (offB[j % 4] + j * 16 * kRowBytes));
-                fb[j][1] = *reinterpret_cast<const uint4*>(b + ((offB[j % 4] ^ 16u) + j * 16 * kRowBytes));
-            }
-#pragma unroll
-            for (int i = 0; i < TM; ++i) {
-                uint4 fa[2];
-                fa[0] = *reinterpret_cast<const uint4*>(a + (offA[i % 4] + i * 16 * kRowBytes));
-                fa[1] = *reinterpret_cast<const uint4*>(a + ((offA[i % 4] ^ 16u) + i * 16 * kRowBytes));
-#pragma unroll
-                for (int j = 0; j < TN; ++j) {
-                    Mfma<KA, KB>::Run(fa, fb[j], acc[i][j]);
-                }
-            }
-        }
+        KBlockMfma<KA, KB, TM, TN>(sA[cur], sB[cur], offA, offB, acc);
         if (more) {
             la.StoreFast(sA[cur ^ 1], q.aK);
             lb.StoreFast(sB[cur ^ 1], q.bK);
@@ -775,9 +796,111 @@ __device__ __forceinline__ void GemmTile(const TileJob& q, int m0, int n0, uint8
     StoreTileC<OK, TM, TN, kFp8 && OK != kOutF16A>(q, acc, m0 + wm, n0 + wn, lane);
 }
 
+// GemmTile with DEPTH k-blocks in flight: register slots hold the global loads
+// of k-blocks kb+1 .. kb+DEPTH while the MFMAs consume k-block kb from LDS, so
+// a workgroup whose k loop is short on work per k-block (small tiles, one
+// workgroup per compute unit) pays the memory latency about once instead of
+// once per k-block.  Same LDS layout and MFMA sequence as GemmTile: the results
+// are bit-identical.  2- and 4-byte elements.
+template <int KA, int KB, int OK, int BM, int BN, int DEPTH>
+__device__ __forceinline__ void GemmTileDeep(const TileJob& q, int m0, int n0, uint8_t (*sA)[BM * kRowBytes],
+                                             uint8_t (*sB)[BN * kRowBytes])
+{
+    constexpr int NT = kThreads;
+    constexpr int EA = KindInfo<KA>::bytes;
+    static_assert(EA == KindInfo<KB>::bytes && (EA == 4 || EA == 2), "2- and 4-byte operands");
+    constexpr int kK = kRowBytes / EA;
+    constexpr int WM = BM / 2;
+    constexpr int WN = BN / 2;
+    constexpr int TM = WM / 16;
+    constexpr int TN = WN / 16;
+    using LA = TileLoader<BM, EA, NT>;
+    using LB = TileLoader<BN, EA, NT>;
+
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = t >> 6;
+    const int wm = (wave % 2) * WM;
+    const int wn = (wave / 2) * WN;
+
+    LA la;
+    LB lb;
+    la.Setup(q.A, q.lda, q.aK, m0);
+    lb.Setup(q.B, q.ldb, q.bK, n0);
+    const int fullK = q.k / kK;
+    const bool interiorA = q.vecA && m0 + BM <= q.m;
+    const bool interiorB = q.vecB && n0 + BN <= q.n;
+
+    f32x4 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    uint4 ra[DEPTH][LA::kPerThread];
+    uint4 rb[DEPTH][LB::kPerThread];
+    const int nk = (q.k + kK - 1) / kK;
+    auto fetch = [&](int kb, uint4 (&da)[LA::kPerThread], uint4 (&db)[LB::kPerThread]) {
+        if (interiorA && kb < fullK) {
+            la.FetchFullInto(kb, da);
+        } else {
+            la.FetchInto(q.A, q.lda, q.aK, q.vecA, m0, q.m, kb * kK, q.k, da);
+        }
+        if (interiorB && kb < fullK) {
+            lb.FetchFullInto(kb, db);
+        } else {
+            lb.FetchInto(q.B, q.ldb, q.bK, q.vecB, n0, q.n, kb * kK, q.k, db);
+        }
+    };
+    dev::StaticFor<0, DEPTH>([&](auto sc) {
+        constexpr int sl = decltype(sc)::value;
+        if (sl < nk) {
+            fetch(sl, ra[sl], rb[sl]);
+        }
+    });
+    if (nk > 0) {
+        la.StoreFastFrom(sA[0], q.aK, ra[0]);
+        lb.StoreFastFrom(sB[0], q.bK, rb[0]);
+    }
+    __syncthreads();
+
+    const int fr = lane & 15;
+    const int fq = (lane >> 4) * 32;
+    uint32_t offA[4], offB[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        offA[i] = SwizzledOffset(wm + i * 16 + fr, fq) - i * 16 * kRowBytes;
+        offB[i] = SwizzledOffset(wn + i * 16 + fr, fq) - i * 16 * kRowBytes;
+    }
+
+    // slot kb % DEPTH holds k-block kb until it is stored to LDS (one
+    // iteration before kb is consumed), then takes k-block kb + DEPTH
+    for (int kb0 = 0; kb0 < nk; kb0 += DEPTH) {
+        dev::StaticFor<0, DEPTH>([&](auto sc) {
+            constexpr int sl = decltype(sc)::value;
+            constexpr int nx = (sl + 1) % DEPTH;
+            const int kb = kb0 + sl;
+            if (kb < nk) {
+                if (kb + DEPTH < nk) {
+                    fetch(kb + DEPTH, ra[sl], rb[sl]);
+                }
+                KBlockMfma<KA, KB, TM, TN>(sA[kb & 1], sB[kb & 1], offA, offB, acc);
+                if (kb + 1 < nk) {
+                    la.StoreFastFrom(sA[(kb + 1) & 1], q.aK, ra[nx]);
+                    lb.StoreFastFrom(sB[(kb + 1) & 1], q.bK, rb[nx]);
+                }
+                __syncthreads();
+            }
+        });
+    }
+
+    StoreTileC<OK, TM, TN, false>(q, acc, m0 + wm, n0 + wn, lane);
+}
+
 // One problem (or a uniform batch through pointer arrays): one workgroup per
 // tile, WGM x WGN waves.
-template <int KA, int KB, int OK, int BM, int BN, int WGM = 2, int WGN = 2>
+// DEPTH > 0: GemmTileDeep with that many k-blocks in flight (2 x 2 waves).
+template <int KA, int KB, int OK, int BM, int BN, int WGM = 2, int WGN = 2, int DEPTH = 0>
 __global__ __launch_bounds__(kWaveSize * WGM * WGN) void GemmKernel(GemmArgs p)
 {
     __shared__ __attribute__((aligned(16))) uint8_t sA[2][BM * kRowBytes];
@@ -816,7 +939,11 @@ __global__ __launch_bounds__(kWaveSize * WGM * WGN) void GemmKernel(GemmArgs p)
     if (p.scaleB != nullptr) alpha *= *p.scaleB;
     q.alpha = alpha;
     q.beta = p.beta;
-    GemmTile<KA, KB, OK, BM, BN, WGM, WGN>(q, tm * BM, tn * BN, sA, sB);
+    if constexpr (DEPTH > 0) {
+        GemmTileDeep<KA, KB, OK, BM, BN, DEPTH>(q, tm * BM, tn * BN, sA, sB);
+    } else {
+        GemmTile<KA, KB, OK, BM, BN, WGM, WGN>(q, tm * BM, tn * BN, sA, sB);
+    }
 }
 
 // Problem record of the persistent grouped kernel (built on the host).
@@ -981,6 +1108,23 @@ inline void LaunchGemmTyped(hipStream_t st, GemmArgs p, int batch, int tile, int
         hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 128, 128>), dim3(p.tilesM * p.tilesN * batch), dim3(kThreads),
                            0, st, p);
         return;
+    }
+    if constexpr (EB == 4) {
+        // 32 x 32 tiles with eight k-blocks in flight, 64 x 64 with four
+        if (tile == 32) {
+            p.tilesM = (p.m + 31) / 32;
+            p.tilesN = (p.n + 31) / 32;
+            hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 32, 32, 2, 2, 8>), dim3(p.tilesM * p.tilesN * batch),
+                               dim3(kThreads), 0, st, p);
+            return;
+        }
+        if (tile == 65) {
+            p.tilesM = (p.m + 63) / 64;
+            p.tilesN = (p.n + 63) / 64;
+            hipLaunchKernelGGL((GemmKernel<KA, KB, OK, 64, 64, 2, 2, 4>), dim3(p.tilesM * p.tilesN * batch),
+                               dim3(kThreads), 0, st, p);
+            return;
+        }
     }
     p.tilesM = (p.m + 63) / 64;
     p.tilesN = (p.n + 63) / 64;

@@ -106,7 +106,7 @@ __global__ __launch_bounds__(kThreads, 3) void GesvFactorKernel(const MixedSyste
     const int t = threadIdx.x;
     for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
         const MixedSystem s = systems[idx];
-        if (onlyMarked != 0 && s.state[1] == 0) {
+        if (onlyMarked != 0 && dev::G(s.state)[1] == 0) {
             continue;
         }
         const int n = s.n;
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(kThreads, 3) void GesvFactorKernel(const MixedSyste
             __syncthreads();
         }
         if (t == 0) {
-            s.state[0] = failed;
+            dev::G(s.state)[0] = failed;
         }
         __syncthreads();
     }
@@ -162,14 +162,14 @@ __device__ __forceinline__ void FactoredSolve(const MixedSystem& s, RefineShared
     const int wave = t / kWave;
     const int r16 = lane & 15;
     const int grp = lane >> 4;
-    const float* LU = s.LU;
+    const auto LU = dev::G(s.LU);
     const int blocks = (n + kNb - 1) / kNb;
     for (int p = 0; p < blocks; ++p) {
         const int k0 = p * kNb;
         const int rows = n - k0;
         {
             // row interchanges of this panel: gather through the map in registers, write back
-            const int* map = s.rowmap + RowMapOffset(n, p);
+            const auto map = dev::G(s.rowmap + RowMapOffset(n, p));
             const int src = t < rows ? map[t] : 0;
             float g[kMaxRhs];
             if (t < rows) {
@@ -194,7 +194,7 @@ __device__ __forceinline__ void FactoredSolve(const MixedSystem& s, RefineShared
         __syncthreads();
         if (wave == 0) {
             // diagonal block: y = inv(L11) b (rows past n are zero in b, identity in the inverse)
-            const float* Li = s.dinv + static_cast<size_t>(p) * kInvFloats;
+            const auto Li = dev::G(s.dinv + static_cast<size_t>(p) * kInvFloats);
             f32x4 y[2];
             dev::StaticFor<0, 2>([&](auto bc) {
                 constexpr int b = decltype(bc)::value;
@@ -327,7 +327,7 @@ __device__ __forceinline__ double Residual(const MixedSystem& s, RefineShared& s
     double rs = 0.0;
     dev::StaticFor<0, kMaxRhs>([&](auto cc) {
         constexpr int c = decltype(cc)::value;
-        r[c] = (t < n && c < nrhs) ? static_cast<double>(s.B[t + static_cast<size_t>(c) * s.ldb]) : 0.0;
+        r[c] = (t < n && c < nrhs) ? static_cast<double>(dev::G(s.B)[t + static_cast<size_t>(c) * s.ldb]) : 0.0;
     });
     if (t < n) {
         // eight columns of A per step, the next eight loaded while these are used
@@ -335,13 +335,13 @@ __device__ __forceinline__ double Residual(const MixedSystem& s, RefineShared& s
         float cur[kCols], nxt[kCols];
         dev::StaticFor<0, kCols>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
-            cur[j] = j < n ? s.A[t + static_cast<size_t>(j) * s.lda] : 0.0f;
+            cur[j] = j < n ? dev::G(s.A)[t + static_cast<size_t>(j) * s.lda] : 0.0f;
         });
         for (int j0 = 0; j0 < n; j0 += kCols) {
             dev::StaticFor<0, kCols>([&](auto jc) {
                 constexpr int j = decltype(jc)::value;
                 const int col = j0 + kCols + j;
-                nxt[j] = col < n ? s.A[t + static_cast<size_t>(col) * s.lda] : 0.0f;
+                nxt[j] = col < n ? dev::G(s.A)[t + static_cast<size_t>(col) * s.lda] : 0.0f;
             });
             dev::StaticFor<0, kCols>([&](auto jc) {
                 constexpr int j = decltype(jc)::value;
@@ -374,8 +374,8 @@ __device__ __forceinline__ double Residual(const MixedSystem& s, RefineShared& s
         if (c < nrhs) {
             const double rv = WaveMaxD(fabs(r[c]));
             const double xv = WaveMaxD(t < n ? fabs(sh.x[t][c]) : 0.0);
-            const double bv = WaveMaxD(t < n ? fabs(static_cast<double>(s.B[t + static_cast<size_t>(c) * s.ldb]))
-                                             : 0.0);
+            const double bv =
+                WaveMaxD(t < n ? fabs(static_cast<double>(dev::G(s.B)[t + static_cast<size_t>(c) * s.ldb])) : 0.0);
             if (lane == 0) {
                 sh.red[0][wave][c] = rv;
                 sh.red[1][wave][c] = xv;
@@ -417,22 +417,22 @@ __global__ __launch_bounds__(kThreads, 3) void GesvRefineKernel(const MixedSyste
     const int t = threadIdx.x;
     for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
         const MixedSystem s = systems[idx];
-        if (onlyMarked != 0 && s.state[1] == 0) {
+        if (onlyMarked != 0 && dev::G(s.state)[1] == 0) {
             continue;
         }
         const int n = s.n;
         const int nrhs = s.nrhs;
-        const int status = s.state[0];
+        const int status = dev::G(s.state)[0];
         if (status != 0 && mayMark != 0) {
             if (t == 0) {
-                s.state[1] = 1;
+                dev::G(s.state)[1] = 1;
             }
             continue;
         }
         if (status != 0) {
             for (int c = 0; c < nrhs; ++c) {
                 for (int i = t; i < n; i += kThreads) {
-                    s.X[i + static_cast<size_t>(c) * s.ldx] = __builtin_nanf("");
+                    dev::G(s.X)[i + static_cast<size_t>(c) * s.ldx] = __builtin_nanf("");
                 }
             }
             if (t == 0) {
@@ -440,14 +440,14 @@ __global__ __launch_bounds__(kThreads, 3) void GesvRefineKernel(const MixedSyste
                 if (berrOut != nullptr) berrOut[s.index] = __builtin_inff();
                 if (infoOut != nullptr) infoOut[s.index] = status;
                 if (fallbackOut != nullptr) fallbackOut[s.index] = onlyMarked;
-                s.state[1] = 0;
+                dev::G(s.state)[1] = 0;
             }
             continue;
         }
         // x0 = LU \ b (rows >= n and columns >= nrhs of bs and x are zero)
         dev::StaticFor<0, kMaxRhs>([&](auto cc) {
             constexpr int c = decltype(cc)::value;
-            sh.bs[t][c] = (t < n && c < nrhs) ? s.B[t + static_cast<size_t>(c) * s.ldb] : 0.0f;
+            sh.bs[t][c] = (t < n && c < nrhs) ? dev::G(s.B)[t + static_cast<size_t>(c) * s.ldb] : 0.0f;
         });
         __syncthreads();
         FactoredSolve(s, sh);
@@ -500,14 +500,14 @@ __global__ __launch_bounds__(kThreads, 3) void GesvRefineKernel(const MixedSyste
         }
         if (!converged && mayMark != 0) {
             if (t == 0) {
-                s.state[1] = 1;
+                dev::G(s.state)[1] = 1;
             }
             __syncthreads();
             continue;
         }
         if (t < n) {
             for (int c = 0; c < nrhs; ++c) {
-                s.X[t + static_cast<size_t>(c) * s.ldx] = static_cast<float>(sh.x[t][c]);
+                dev::G(s.X)[t + static_cast<size_t>(c) * s.ldx] = static_cast<float>(sh.x[t][c]);
             }
         }
         if (t == 0) {
@@ -515,7 +515,7 @@ __global__ __launch_bounds__(kThreads, 3) void GesvRefineKernel(const MixedSyste
             if (berrOut != nullptr) berrOut[s.index] = static_cast<float>(worst);
             if (infoOut != nullptr) infoOut[s.index] = 0;
             if (fallbackOut != nullptr) fallbackOut[s.index] = onlyMarked;
-            s.state[1] = 0;
+            dev::G(s.state)[1] = 0;
         }
         __syncthreads();
     }

@@ -149,12 +149,13 @@ __device__ __forceinline__ int FactorPanel(const PanelView& pv, int k0, int* gma
     const int r = wave * kWave + lane;
     const bool valid = r < rows;
     const bool active = wave * kWave < rows;  // the wave holds panel rows
-    float* LU = pv.LU;
+    const auto LU = dev::G(pv.LU);
+    const auto src = dev::G(pv.src);
 
     float a[kNb];
     dev::StaticFor<0, kNb>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
-        a[j] = (valid && j < w) ? pv.src[(k0 + r) + static_cast<size_t>(k0 + j) * sld] : 0.0f;
+        a[j] = (valid && j < w) ? src[(k0 + r) + static_cast<size_t>(k0 + j) * sld] : 0.0f;
     });
     if (!active && lane == 0) {
         sh.cand64[0][wave] = 0;
@@ -261,7 +262,7 @@ __device__ __forceinline__ int FactorPanel(const PanelView& pv, int k0, int* gma
         const int p = pos;
         sh.rowmap[p] = r;
         if (gmap != nullptr) {
-            gmap[p] = r;
+            dev::G(gmap)[p] = r;
         }
         dev::StaticFor<0, kNb>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
@@ -341,7 +342,7 @@ __device__ __forceinline__ void PanelInverses(const PanelView& pv, int k0, int w
             constexpr int q = decltype(qc)::value;
             sh.linv[q][j] = x[q];
             if (g != nullptr) {
-                g[q + kNb * j] = x[q];
+                dev::G(g)[q + kNb * j] = x[q];
             }
         });
     }
@@ -363,7 +364,7 @@ __device__ __forceinline__ void GatherTile(const PanelView& pv, int k0, int c0, 
     const int rows = n - k0;
     const int r16 = lane & 15;
     const int cq = (lane >> 4) * 4;
-    const float* LU = pv.src;
+    const auto LU = dev::G(pv.src);
     const int tiles = (rows + 15) / 16;
     dev::StaticFor<0, kMaxTiles>([&](auto tc) {
         constexpr int t = decltype(tc)::value;
@@ -392,7 +393,7 @@ __device__ __forceinline__ void UpdateTile(const PanelView& pv, int k0, int c0, 
     const int r16 = lane & 15;
     const int grp = lane >> 4;
     const int cq = grp * 4;
-    float* LU = pv.LU;
+    const auto LU = dev::G(pv.LU);
     const int tiles = (rows + 15) / 16;
 
     // U12 = inv(L11) A12 (top 32 rows) on the FP32 matrix instruction, as

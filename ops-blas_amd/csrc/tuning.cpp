@@ -89,7 +89,8 @@ bool Valid(int op, int bucket, Choice c)
     switch (op) {
         case kOpGemm: {
             const int cls = bucket / (kGemmSizeBuckets * kGemmKBuckets);
-            const bool tileOk = c.a == 64 || c.a == 128 || (c.a == 256 && cls != kClassF32);
+            const bool tileOk = c.a == 64 || c.a == 128 || (c.a == 256 && cls != kClassF32) ||
+                                ((c.a == 32 || c.a == 65) && cls == kClassF32);
             return tileOk && c.b >= 1 && c.b <= 64;
         }
         case kOpGrouped:

@@ -16,7 +16,8 @@ namespace aclblas {
 namespace tune {
 
 enum Op : int {
-    kOpGemm = 0,     // a = tile edge (64, 128, 256), b = tile rows per L2 group (1..64)
+    kOpGemm = 0,     // a = tile edge (64, 128, 256; FP32 also 32 and 65 = 64 with deep prefetch),
+                     // b = tile rows per L2 group (1..64)
     kOpGrouped = 1,  // a = tile edge (64, 128), b = workgroups per CU of the persistent grid (1..4)
     kOpMixed = 2,    // a = factor workgroups per CU (1..4), b = solve workgroups per CU (1..4); class 0: one-
                      // and two-plane BF16 trailing updates, 1: FP32 and three planes

@@ -46,6 +46,34 @@ def _all_choices_equal(h, op, choices, run, outputs):
 GEMM_CHOICES = [(t, g) for t in (64, 128, 256) for g in (1, 3, 8, 64)]
 
 
+@pytest.mark.parametrize("trans", ["NN", "NT", "TN", "TT"])
+@pytest.mark.parametrize("shape", [(300, 257, 129), (33, 65, 40), (64, 64, 1000), (97, 31, 290)])
+def test_fp32_deep_prefetch_tiles_agree(h, trans, shape):
+    """FP32 tiles 32 (eight k-blocks of loads in flight) and 65 (64 x 64, four in flight) run the same
+    MFMA sequence per output as the 64 and 128 tiles: bit-identical results, also when the k loop is
+    shorter than the prefetch depth (k = 40) or wraps the register ring several times (k = 1000)."""
+    m, n, k = shape
+    dev = torch.device("cuda")
+    gen = torch.Generator(device="cpu").manual_seed(m * 7 + n * 3 + k)
+    ta, tb = trans[0] == "T", trans[1] == "T"
+    a = (torch.rand(m * k, generator=gen) * 2 - 1).to(dev)
+    b = (torch.rand(k * n, generator=gen) * 2 - 1).to(dev)
+    c0 = torch.randn(m * n, generator=gen).to(dev)
+    c = c0.clone()
+    lda, ldb = (k if ta else m), (n if tb else k)
+
+    def run():
+        c.copy_(c0)
+        assert B.sgemm(h, B.OP_T if ta else B.OP_N, B.OP_T if tb else B.OP_N, m, n, k, 1.25, a.data_ptr(), lda,
+                       b.data_ptr(), ldb, -0.5, c.data_ptr(), m) == 0
+    _all_choices_equal(h, B.TUNE_GEMM, [(64, 8), (128, 8), (32, 1), (32, 8), (65, 8)], run, lambda: [c])
+    ref = (c0.cpu().double().reshape(n, m).T * -0.5 +
+           1.25 * (a.cpu().double().reshape(m, k) if ta else a.cpu().double().reshape(k, m).T) @
+           (b.cpu().double().reshape(k, n) if tb else b.cpu().double().reshape(n, k).T))
+    got = c.cpu().double().reshape(n, m).T
+    assert torch.allclose(got, ref, rtol=0, atol=1e-4 * (k ** 0.5))
+
+
 @pytest.mark.parametrize("kind", ["f32", "bf16", "f8", "f8s"])
 @pytest.mark.parametrize("shape", [(300, 257, 129), (520, 130, 700)])
 def test_gemm_choices_agree(h, kind, shape):
@@ -62,7 +90,7 @@ def test_gemm_choices_agree(h, kind, shape):
             c.copy_(c0)
             assert B.sgemm(h, B.OP_T, B.OP_N, m, n, k, 1.5, a.data_ptr(), k, b.data_ptr(), k, 0.5, c.data_ptr(),
                            m) == 0
-        choices = [x for x in GEMM_CHOICES if x[0] != 256]
+        choices = [x for x in GEMM_CHOICES if x[0] != 256] + [(t, g) for t in (32, 65) for g in (1, 8)]
     elif kind == "bf16":
         a = torch.randn(m * k, generator=gen).to(dev).to(torch.bfloat16)
         b = torch.randn(k * n, generator=gen).to(dev).to(torch.bfloat16)
