@@ -149,6 +149,32 @@ struct RefineShared {
 
 __device__ __forceinline__ double MaxD(double a, double b) { return b > a ? b : a; }
 
+// ok ? p[idx] : 0 with an unconditional load (of p[0] when !ok, which is in
+// bounds wherever this is used): the compiler can then issue the loads of a
+// sequence together, where a load under a lane-dependent condition is issued
+// inside a branch and waited for there.
+// Keeps the compiler's scheduler from moving instructions across this point:
+// placed after a group of loads, it keeps them issued together ahead of their
+// uses (under register pressure the scheduler otherwise pairs each load with
+// its first use and waits for it there).
+__device__ __forceinline__ void LoadsBeforeUses() { __builtin_amdgcn_sched_barrier(0); }
+
+template <typename T, typename P>
+__device__ __forceinline__ T LoadOr0(P p, size_t idx, bool ok)
+{
+    const T v = p[ok ? idx : 0];
+    return ok ? v : T(0);
+}
+
+// p[idx] when ok, else p[0] (in bounds wherever this is used; the caller
+// replaces it by 0 after LoadsBeforeUses, so the loads do not wait one by one
+// for a select).
+template <typename P>
+__device__ __forceinline__ float LoadAt(P p, size_t idx, bool ok)
+{
+    return p[ok ? idx : 0];
+}
+
 // bs <- LU^-1 bs: per panel the row map, inv(L11) on the diagonal block and
 // the block's L21 below it; then from the bottom each diagonal block of U by
 // substitution and its U column block above it.  FP32 matrix instruction: a
@@ -170,7 +196,7 @@ __device__ __forceinline__ void FactoredSolve(const MixedSystem& s, RefineShared
         {
             // row interchanges of this panel: gather through the map in registers, write back
             const auto map = dev::G(s.rowmap + RowMapOffset(n, p));
-            const int src = t < rows ? map[t] : 0;
+            const int src = LoadOr0<int>(map, t, t < rows);
             float g[kMaxRhs];
             if (t < rows) {
                 dev::StaticFor<0, kMaxRhs / 4>([&](auto cc) {
@@ -195,15 +221,23 @@ __device__ __forceinline__ void FactoredSolve(const MixedSystem& s, RefineShared
         if (wave == 0) {
             // diagonal block: y = inv(L11) b (rows past n are zero in b, identity in the inverse)
             const auto Li = dev::G(s.dinv + static_cast<size_t>(p) * kInvFloats);
+            // operands first, then the MFMA chains (the loads issue together)
+            float li[2][kNb / 4];
+            dev::StaticFor<0, 2>([&](auto bc) {
+                constexpr int b = decltype(bc)::value;
+                dev::StaticFor<0, kNb / 4>([&](auto sc) {
+                    constexpr int st = decltype(sc)::value;
+                    li[b][st] = Li[(16 * b + r16) + kNb * (4 * st + grp)];
+                });
+            });
+            LoadsBeforeUses();
             f32x4 y[2];
             dev::StaticFor<0, 2>([&](auto bc) {
                 constexpr int b = decltype(bc)::value;
                 f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
                 dev::StaticFor<0, kNb / 4>([&](auto sc) {
                     constexpr int st = decltype(sc)::value;
-                    const int k = 4 * st + grp;
-                    v = __builtin_amdgcn_mfma_f32_16x16x4f32(Li[(16 * b + r16) + kNb * k], sh.bs[k0 + k][r16], v, 0,
-                                                             0, 0);
+                    v = __builtin_amdgcn_mfma_f32_16x16x4f32(li[b][st], sh.bs[k0 + 4 * st + grp][r16], v, 0, 0, 0);
                 });
                 y[b] = v;
             });
@@ -223,12 +257,17 @@ __device__ __forceinline__ void FactoredSolve(const MixedSystem& s, RefineShared
             for (int blk = wave; blk < nb16; blk += kWaves) {
                 const int r0 = k0 + kNb + 16 * blk;
                 const int row = r0 + r16;
+                float a[kNb / 4];
+                dev::StaticFor<0, kNb / 4>([&](auto sc) {
+                    constexpr int st = decltype(sc)::value;
+                    a[st] = LoadAt(LU, row + static_cast<size_t>(k0 + 4 * st + grp) * n, row < n);
+                });
+                LoadsBeforeUses();
                 f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
                 dev::StaticFor<0, kNb / 4>([&](auto sc) {
                     constexpr int st = decltype(sc)::value;
-                    const int k = 4 * st + grp;
-                    const float a = row < n ? LU[row + static_cast<size_t>(k0 + k) * n] : 0.0f;
-                    v = __builtin_amdgcn_mfma_f32_16x16x4f32(a, sh.bs[k0 + k][r16], v, 0, 0, 0);
+                    v = __builtin_amdgcn_mfma_f32_16x16x4f32(row < n ? a[st] : 0.0f, sh.bs[k0 + 4 * st + grp][r16], v,
+                                                             0, 0, 0);
                 });
                 dev::StaticFor<0, 4>([&](auto jc) {
                     constexpr int j = decltype(jc)::value;
@@ -253,11 +292,16 @@ __device__ __forceinline__ void FactoredSolve(const MixedSystem& s, RefineShared
             float x[kMaxRhs / kWaves];
             dev::StaticFor<0, kNb>([&](auto jc) {
                 constexpr int j = decltype(jc)::value;
-                u[j] = (valid && j >= lane && j < w) ? LU[(r0 + lane) + static_cast<size_t>(r0 + j) * n] : 0.0f;
+                u[j] = LoadAt(LU, (r0 + lane) + static_cast<size_t>(r0 + j) * n, valid && j >= lane && j < w);
             });
             dev::StaticFor<0, kMaxRhs / kWaves>([&](auto cc) {
                 constexpr int ci = decltype(cc)::value;
                 x[ci] = valid ? sh.bs[r0 + lane][ci * kWaves + wave] : 0.0f;
+            });
+            LoadsBeforeUses();
+            dev::StaticFor<0, kNb>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                u[j] = (valid && j >= lane && j < w) ? u[j] : 0.0f;
             });
             dev::StaticFor<0, kNb>([&](auto jc) {
                 constexpr int j = kNb - 1 - decltype(jc)::value;
@@ -288,12 +332,18 @@ __device__ __forceinline__ void FactoredSolve(const MixedSystem& s, RefineShared
         const int nb16 = r0 / 16;
         for (int blk = wave; blk < nb16; blk += kWaves) {
             const int row = 16 * blk + r16;
-            f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+            float a[kNb / 4];
             dev::StaticFor<0, kNb / 4>([&](auto sc) {
                 constexpr int st = decltype(sc)::value;
                 const int k = 4 * st + grp;
-                const float a = r0 + k < n ? LU[row + static_cast<size_t>(r0 + k) * n] : 0.0f;
-                v = __builtin_amdgcn_mfma_f32_16x16x4f32(a, sh.bs[r0 + k][r16], v, 0, 0, 0);
+                a[st] = LoadAt(LU, row + static_cast<size_t>(r0 + k) * n, r0 + k < n);
+            });
+            LoadsBeforeUses();
+            f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+            dev::StaticFor<0, kNb / 4>([&](auto sc) {
+                constexpr int st = decltype(sc)::value;
+                v = __builtin_amdgcn_mfma_f32_16x16x4f32(r0 + 4 * st + grp < n ? a[st] : 0.0f,
+                                                         sh.bs[r0 + 4 * st + grp][r16], v, 0, 0, 0);
             });
             dev::StaticFor<0, 4>([&](auto jc) {
                 constexpr int j = decltype(jc)::value;
@@ -312,85 +362,164 @@ __device__ __forceinline__ double WaveMaxD(double v)
     return v;
 }
 
-// r = b - A x in FP64 (A read in FP32, x from LDS, one row per thread: row t
-// of r stays in the caller's registers); the worst normwise backward error
-// over the right-hand sides is returned (uniform).  With withNorm, |A|_inf is
-// computed into sh.anorm first.
-__device__ __forceinline__ double Residual(const MixedSystem& s, RefineShared& sh, bool withNorm,
-                                           double (&r)[kMaxRhs])
+// Thread t holds rows 4 (t / 4) .. +3 and right-hand sides 4 (t % 4) .. +3 of
+// the residual.
+constexpr int kResRows = 4;
+constexpr int kResRhs = 4;
+typedef double ResidualTile[kResRows][kResRhs];
+
+// Lane I of each aligned group of four lanes, to all four (DPP quad_perm
+// [I, I, I, I]; every lane is active where this is used).
+template <int I>
+__device__ __forceinline__ float QuadBroadcast(float v)
+{
+    return __uint_as_float(dev::Dpp<I * 0x55>(__float_as_uint(v)));
+}
+
+__device__ __forceinline__ double MaxOverQuadLanes(double v)
+{
+    // lanes with the same t % 4 (xor over lane bits 2..5)
+    for (int off = 4; off < kWave; off <<= 1) {
+        v = MaxD(v, __shfl_xor(v, off, kWave));
+    }
+    return v;
+}
+
+// r = b - A x in FP64 (A read in FP32, x from LDS) into the caller's registers
+// (ResidualTile layout); the worst normwise backward error over the
+// right-hand sides is returned (uniform).  With withNorm, |A|_inf is computed
+// into sh.anorm first.
+// Per column of A a lane loads one value (row 4 (t / 4) + t % 4) and its group
+// of four lanes shares the four rows by DPP; each lane then reads 32 bytes of
+// x for 16 FMAs: a quarter of the LDS bytes per FMA of one row per thread with
+// all sixteen right-hand sides, whose broadcast reads bound the LDS.
+template <bool NORM>
+__device__ __forceinline__ void ResidualColumns(const MixedSystem& s, const RefineShared& sh, int row0, int q,
+                                                ResidualTile& r, double (&rs)[kResRows])
+{
+    const int n = s.n;
+    const int t = threadIdx.x;
+    const int myRow = row0 + (t & 3);
+    const bool rowOk = myRow < n;
+    const auto A = dev::G(s.A);
+    const size_t lda = static_cast<size_t>(s.lda);
+    // eight columns per step, the next eight loaded while these are used
+    constexpr int kCols = 8;
+    float cur[kCols], nxt[kCols];
+    dev::StaticFor<0, kCols>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        cur[j] = LoadAt(A, myRow + static_cast<size_t>(j) * lda, rowOk && j < n);
+    });
+    for (int j0 = 0; j0 < n; j0 += kCols) {
+        dev::StaticFor<0, kCols>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            const int col = j0 + kCols + j;
+            nxt[j] = LoadAt(A, myRow + static_cast<size_t>(col) * lda, rowOk && col < n);
+        });
+        LoadsBeforeUses();
+        dev::StaticFor<0, kCols>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            // x rows past n are zero, so columns past n add nothing (j0 + j < kMaxN: n <= kMaxN, kCols | kMaxN)
+            const double2 x01 = *reinterpret_cast<const double2*>(&sh.x[j0 + j][4 * q]);
+            const double2 x23 = *reinterpret_cast<const double2*>(&sh.x[j0 + j][4 * q + 2]);
+            const double xv[4] = {x01.x, x01.y, x23.x, x23.y};
+            const float mine = (rowOk && j0 + j < n) ? cur[j] : 0.0f;
+            double a[kResRows];
+            a[0] = static_cast<double>(QuadBroadcast<0>(mine));
+            a[1] = static_cast<double>(QuadBroadcast<1>(mine));
+            a[2] = static_cast<double>(QuadBroadcast<2>(mine));
+            a[3] = static_cast<double>(QuadBroadcast<3>(mine));
+            dev::StaticFor<0, kResRows>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                if constexpr (NORM) {
+                    rs[i] += fabs(a[i]);
+                }
+                dev::StaticFor<0, kResRhs>([&](auto ec) {
+                    constexpr int e = decltype(ec)::value;
+                    r[i][e] = fma(-a[i], xv[e], r[i][e]);
+                });
+            });
+        });
+        dev::StaticFor<0, kCols>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            cur[j] = nxt[j];
+        });
+    }
+}
+
+__device__ __forceinline__ double Residual(const MixedSystem& s, RefineShared& sh, bool withNorm, ResidualTile& r)
 {
     const int n = s.n;
     const int nrhs = s.nrhs;
     const int t = threadIdx.x;
     const int lane = dev::LaneId();
     const int wave = t / kWave;
-    double rs = 0.0;
-    dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-        constexpr int c = decltype(cc)::value;
-        r[c] = (t < n && c < nrhs) ? static_cast<double>(dev::G(s.B)[t + static_cast<size_t>(c) * s.ldb]) : 0.0;
-    });
-    if (t < n) {
-        // eight columns of A per step, the next eight loaded while these are used
-        constexpr int kCols = 8;
-        float cur[kCols], nxt[kCols];
-        dev::StaticFor<0, kCols>([&](auto jc) {
-            constexpr int j = decltype(jc)::value;
-            cur[j] = j < n ? dev::G(s.A)[t + static_cast<size_t>(j) * s.lda] : 0.0f;
+    const int q = t & 3;
+    const int row0 = (t >> 2) * kResRows;
+    const auto Bm = dev::G(s.B);
+    const size_t ldb = static_cast<size_t>(s.ldb);
+    float bf[kResRows][kResRhs];
+    dev::StaticFor<0, kResRhs>([&](auto ec) {
+        constexpr int e = decltype(ec)::value;
+        const int c = kResRhs * q + e;
+        dev::StaticFor<0, kResRows>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            const int row = row0 + i;
+            bf[i][e] = LoadAt(Bm, row + static_cast<size_t>(c) * ldb, row < n && c < nrhs);
         });
-        for (int j0 = 0; j0 < n; j0 += kCols) {
-            dev::StaticFor<0, kCols>([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
-                const int col = j0 + kCols + j;
-                nxt[j] = col < n ? dev::G(s.A)[t + static_cast<size_t>(col) * s.lda] : 0.0f;
-            });
-            dev::StaticFor<0, kCols>([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
-                const double a = static_cast<double>(cur[j]);
-                rs += fabs(a);
-                // x rows past n are zero, so the last partial group needs no check
-                dev::StaticFor<0, kMaxRhs / 2>([&](auto cc) {
-                    constexpr int c = decltype(cc)::value;
-                    if (2 * c < nrhs) {
-                        const double2 xv = *reinterpret_cast<const double2*>(&sh.x[j0 + j][2 * c]);
-                        r[2 * c] = fma(-a, xv.x, r[2 * c]);
-                        r[2 * c + 1] = fma(-a, xv.y, r[2 * c + 1]);
-                    }
-                });
-            });
-            dev::StaticFor<0, kCols>([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
-                cur[j] = nxt[j];
-            });
+    });
+    LoadsBeforeUses();
+    double bmax[kResRhs];
+    dev::StaticFor<0, kResRhs>([&](auto ec) {
+        constexpr int e = decltype(ec)::value;
+        bmax[e] = 0.0;
+        dev::StaticFor<0, kResRows>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            r[i][e] = (row0 + i < n && kResRhs * q + e < nrhs) ? static_cast<double>(bf[i][e]) : 0.0;
+            bmax[e] = MaxD(bmax[e], fabs(r[i][e]));
+        });
+    });
+    double rs[kResRows] = {0.0, 0.0, 0.0, 0.0};
+    if (wave * kWave < n) {  // the wave's rows 64 wave .. +63 reach into the system
+        if (withNorm) {
+            ResidualColumns<true>(s, sh, row0, q, r, rs);
+        } else {
+            ResidualColumns<false>(s, sh, row0, q, r, rs);
         }
     }
     if (withNorm) {
-        const double m = WaveMaxD(rs);
+        double m = MaxD(MaxD(rs[0], rs[1]), MaxD(rs[2], rs[3]));
+        m = WaveMaxD(m);
         if (lane == 0) {
             sh.anorm[wave] = m;
         }
     }
-    dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-        constexpr int c = decltype(cc)::value;
-        if (c < nrhs) {
-            const double rv = WaveMaxD(fabs(r[c]));
-            const double xv = WaveMaxD(t < n ? fabs(sh.x[t][c]) : 0.0);
-            const double bv =
-                WaveMaxD(t < n ? fabs(static_cast<double>(dev::G(s.B)[t + static_cast<size_t>(c) * s.ldb])) : 0.0);
-            if (lane == 0) {
-                sh.red[0][wave][c] = rv;
-                sh.red[1][wave][c] = xv;
-                sh.red[2][wave][c] = bv;
-            }
+    dev::StaticFor<0, kResRhs>([&](auto ec) {
+        constexpr int e = decltype(ec)::value;
+        const int c = kResRhs * q + e;
+        double rv = 0.0, xv = 0.0;
+        dev::StaticFor<0, kResRows>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            rv = MaxD(rv, fabs(r[i][e]));
+            xv = MaxD(xv, row0 + i < n ? fabs(sh.x[row0 + i][c]) : 0.0);
+        });
+        rv = MaxOverQuadLanes(rv);
+        xv = MaxOverQuadLanes(xv);
+        const double bv = MaxOverQuadLanes(bmax[e]);
+        if (lane < 4) {
+            sh.red[0][wave][c] = rv;
+            sh.red[1][wave][c] = xv;
+            sh.red[2][wave][c] = bv;
         }
     });
     __syncthreads();
     if (t < nrhs) {
         double an = 0.0, rn = 0.0, xn = 0.0, bn = 0.0;
-        for (int q = 0; q < kWaves; ++q) {
-            an = MaxD(an, sh.anorm[q]);
-            rn = MaxD(rn, sh.red[0][q][t]);
-            xn = MaxD(xn, sh.red[1][q][t]);
-            bn = MaxD(bn, sh.red[2][q][t]);
+        for (int w = 0; w < kWaves; ++w) {
+            an = MaxD(an, sh.anorm[w]);
+            rn = MaxD(rn, sh.red[0][w][t]);
+            xn = MaxD(xn, sh.red[1][w][t]);
+            bn = MaxD(bn, sh.red[2][w][t]);
         }
         const double den = an * xn + bn;
         sh.berr[t] = den > 0.0 ? rn / den : (rn > 0.0 ? __builtin_inf() : 0.0);
@@ -445,10 +574,18 @@ __global__ __launch_bounds__(kThreads, 3) void GesvRefineKernel(const MixedSyste
             continue;
         }
         // x0 = LU \ b (rows >= n and columns >= nrhs of bs and x are zero)
-        dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-            constexpr int c = decltype(cc)::value;
-            sh.bs[t][c] = (t < n && c < nrhs) ? dev::G(s.B)[t + static_cast<size_t>(c) * s.ldb] : 0.0f;
-        });
+        {
+            float b0[kMaxRhs];
+            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
+                constexpr int c = decltype(cc)::value;
+                b0[c] = LoadAt(dev::G(s.B), t + static_cast<size_t>(c) * s.ldb, t < n && c < nrhs);
+            });
+            LoadsBeforeUses();
+            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
+                constexpr int c = decltype(cc)::value;
+                sh.bs[t][c] = (t < n && c < nrhs) ? b0[c] : 0.0f;
+            });
+        }
         __syncthreads();
         FactoredSolve(s, sh);
         dev::StaticFor<0, kMaxRhs>([&](auto cc) {
@@ -459,7 +596,7 @@ __global__ __launch_bounds__(kThreads, 3) void GesvRefineKernel(const MixedSyste
         int iter = 0;
         double worst = 0.0, prev = 0.0;
         bool converged = false;
-        double r[kMaxRhs];
+        ResidualTile r;
         for (;;) {
             worst = Residual(s, sh, iter == 0, r);
             if (iter > 0 && !(worst <= prev)) {
@@ -481,10 +618,12 @@ __global__ __launch_bounds__(kThreads, 3) void GesvRefineKernel(const MixedSyste
             if (converged || stalled) {
                 break;
             }
-            // d = LU \ r (FP32), x += d
-            dev::StaticFor<0, kMaxRhs>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                sh.bs[t][c] = static_cast<float>(r[c]);
+            // d = LU \ r (FP32), x += d (the tiles of r cover bs: rows >= n and columns >= nrhs get 0)
+            dev::StaticFor<0, kResRows>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                *reinterpret_cast<float4*>(&sh.bs[(t >> 2) * kResRows + i][kResRhs * (t & 3)]) =
+                    make_float4(static_cast<float>(r[i][0]), static_cast<float>(r[i][1]),
+                                static_cast<float>(r[i][2]), static_cast<float>(r[i][3]));
             });
             __syncthreads();
             FactoredSolve(s, sh);
