@@ -319,7 +319,9 @@ aclblasStatus_t aclblasSetLaunchEvents(aclblasHandle_t handle, void* const event
  *     or for FP32 32 (32 x 32 tiles, eight k-blocks of loads in flight) and
  *     65 (64 x 64 tiles, four k-blocks in flight),
  *     b = tile rows walked together per L2 group, 1..64;
- *   1 grouped GEMM: a = tile edge 64 / 128, b = persistent workgroups per CU 1..4;
+ *   1 grouped GEMM: a = tile edge 64 / 128, b = persistent workgroups per CU 1..4
+ *     (separate buckets for aclblasSgemmGroupedBatched and
+ *     aclblasGemmGroupedBatchedEx);
  *   2 mixed solve: a = factor workgroups per CU 1..4, b = solve workgroups per CU 1..4;
  *   3 batched LU (aclblasSgetrfBatched, n <= 64): a = matrix rows per lane 1 or 2
  *     (2 needs n > 8), b ignored.
@@ -337,11 +339,13 @@ aclblasStatus_t aclblasGetTuningChoice(aclblasHandle_t handle, int op, int bucke
 
 /* Bucket of a problem: op 0 takes cls (0 FP32, 1 BF16/FP16, 2 FP8 under
  * GemmEx, 3 FP8 under GemmExScaled), units = 128 x 128 output tiles over the
- * call and k; op 1 takes units = 128 x 128 tiles over all problems; op 2 takes
+ * call and k; op 1 takes cls (0 FP32 under SgemmGroupedBatched, 1 16-bit or
+ * FP8 under GemmGroupedBatchedEx) and units = 128 x 128 tiles over all
+ * problems; op 2 takes
  * cls = the trailing-update class (0: BF16 and BF16X2, 1: FP32 and BF16X3) and
  * units = number of systems; op 3 takes cls = the order n (1..64) and units =
  * the batch count.
- * cls and k are ignored for op 1, k for ops 2 and 3. */
+ * k is ignored for ops 1, 2 and 3. */
 aclblasStatus_t aclblasTuningBucket(aclblasHandle_t handle, int op, int cls, long long units, int k, int* bucket);
 
 #ifdef __cplusplus

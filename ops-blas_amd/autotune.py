@@ -16,9 +16,10 @@ Buckets and representative problems:
     k = 128, 1024, 4096; candidates tile 64/128/256 (FP32: 32/65/64/128, where
     32 and 65 are 32 x 32 and 64 x 64 tiles with deep load prefetch) x L2
     group height 4/8/16.
-  grouped GEMM, 3 buckets: FP32 problems with m, n, k uniform in steps of 16
-    (16 problems up to 512, 40 up to 768, 64 up to 2048); candidates tile
-    64/128 x 1..4 workgroups per CU.
+  grouped GEMM, 2 x 3 buckets: FP32 (SgemmGroupedBatched) and BF16
+    (GemmGroupedBatchedEx) problems with m, n, k uniform in steps of 16 (16
+    problems up to 512, 40 up to 768, 64 up to 2048); candidates tile 64/128 x
+    1..4 workgroups per CU.
   mixed solve, 8 buckets: BF16 or FP32 updates x 128, 512, 2048, 10000 systems, n 16..256, nrhs
     1..16, cond 1e2..1e6; candidates 1..3 factor x 1..4 solve workgroups per CU.
   batched LU (n <= 64), 16 buckets: n = 16, 32, 64 with batches giving 1, 3,
@@ -203,6 +204,26 @@ class Tuner:
             cands = [(t, w) for t in (64, 128) for w in (1, 2, 3, 4)]
             self._sweep(B.TUNE_GROUPED, bucket, cands, run, lambda: Cs,
                         "%d FP32 problems up to %d (%d tiles)" % (count, top, tiles))
+            del As, Bs, Cs
+
+            # the same shapes in BF16 through aclblasGemmGroupedBatchedEx
+            bucket = self.bucket(B.TUNE_GROUPED, 1, tiles)
+            Ah = [torch.randn(int(mm * kk), device=dev).to(torch.bfloat16) for mm, nn, kk in mnk]
+            Bh = [torch.randn(int(kk * nn), device=dev).to(torch.bfloat16) for mm, nn, kk in mnk]
+            Ch = [torch.empty(int(mm * nn), device=dev, dtype=torch.bfloat16) for mm, nn, kk in mnk]
+            ptr = lambda ts: torch.tensor([x.data_ptr() for x in ts], dtype=torch.int64, device=dev)
+            qa, qb, qc = ptr(Ah), ptr(Bh), ptr(Ch)
+
+            def run_ex():
+                st = B.gemm_grouped_batched_ex(h, [B.OP_N] * count, [B.OP_N] * count, ms_, ns_, ks_, [1.0] * count,
+                                               qa.data_ptr(), B.ACL_BF16, ms_, qb.data_ptr(), B.ACL_BF16, ks_,
+                                               [0.0] * count, qc.data_ptr(), B.ACL_BF16, ms_, [1] * count,
+                                               B.COMPUTE["COMPUTE_32F"])
+                assert st == 0, st
+
+            self._sweep(B.TUNE_GROUPED, bucket, cands, run_ex, lambda: Ch,
+                        "%d BF16 problems up to %d (%d tiles)" % (count, top, tiles))
+            del Ah, Bh, Ch
 
     # ------------------------------------------------------------ mixed solve
     def tune_mixed(self):

@@ -163,7 +163,7 @@ extern "C" aclblasStatus_t aclblasSgemmGroupedBatched(
     for (const GroupedProblem& p : probs) {
         tiles128 += static_cast<long long>((p.m + 127) / 128) * ((p.n + 127) / 128);
     }
-    const tune::Choice c = tune::Get(handle, tune::kOpGrouped, tune::GroupedBucket(tiles128, cus),
+    const tune::Choice c = tune::Get(handle, tune::kOpGrouped, tune::GroupedBucket(0, tiles128, cus),
                                      tune::Choice{kDefaultTile, kDefaultBlocksPerCu});
     const int tile = c.a;
     long long tiles = 0;
@@ -385,7 +385,7 @@ extern "C" aclblasStatus_t aclblasGemmGroupedBatchedEx(
     for (const GroupedProblem& p : probs) {
         tiles128 += static_cast<long long>((p.m + 127) / 128) * ((p.n + 127) / 128);
     }
-    const tune::Choice c = tune::Get(handle, tune::kOpGrouped, tune::GroupedBucket(tiles128, cus),
+    const tune::Choice c = tune::Get(handle, tune::kOpGrouped, tune::GroupedBucket(1, tiles128, cus),
                                      tune::Choice{kDefaultTile, kDefaultBlocksPerCu});
     const int tile = c.a;
     long long tiles = 0;

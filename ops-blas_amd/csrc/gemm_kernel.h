@@ -659,6 +659,56 @@ __device__ __forceinline__ void StoreTileC(const TileJob& q, const f32x4 (&acc)[
     }
 }
 
+// C of a wave's TM x TN MFMA tiles read ahead of the k loop (beta != 0), in
+// the accumulator layout: one round trip less after the last MFMA.
+template <int OK, int TM, int TN>
+__device__ __forceinline__ void PreloadTileC(const TileJob& q, int row0, int col0, int lane, float (&c)[TM][TN][4])
+{
+    const int fr = lane & 15;
+    const int rq = (lane >> 4) * 4;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int col = col0 + j * 16 + fr;
+            const int row = row0 + i * 16 + rq;
+            const bool in = col < q.n && row < q.m;
+            // rows past m or columns past n read element (0, 0) and are never stored
+            const size_t base = in ? static_cast<size_t>(col) * q.ldc + row : 0;
+            LoadC4<OK>(q.C, base, in && q.vecC && row + 4 <= q.m, in ? q.m - row : 1, c[i][j]);
+        }
+    }
+}
+
+// StoreTileC with C already in registers (PreloadTileC).
+template <int OK, int TM, int TN>
+__device__ __forceinline__ void StoreTileCPre(const TileJob& q, const f32x4 (&acc)[TM][TN], const float (&c)[TM][TN][4],
+                                              int row0, int col0, int lane)
+{
+    const float alpha = q.alpha;
+    const float beta = q.beta;
+    const int fr = lane & 15;
+    const int rq = (lane >> 4) * 4;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int col = col0 + j * 16 + fr;
+            const int row = row0 + i * 16 + rq;
+            if (col >= q.n || row >= q.m) {
+                continue;
+            }
+            const size_t base = static_cast<size_t>(col) * q.ldc + row;
+            float v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v[e] = alpha * acc[i][j][e] + beta * c[i][j][e];
+            }
+            StoreC4<OK>(q.C, base, q.vecC && row + 4 <= q.m, q.m - row, v);
+        }
+    }
+}
+
 // The MFMAs of one k-block held in LDS (a, b: the stage's A and B rows; offA,
 // offB: a wave's swizzled fragment offsets).  Every tile shape runs the same
 // instruction sequence per 16 x 16 accumulator, so results do not depend on
@@ -837,6 +887,13 @@ __device__ __forceinline__ void GemmTileDeep(const TileJob& q, int m0, int n0, u
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+    // C first (beta != 0): its loads are the oldest in flight and complete
+    // during the k loop
+    float cpre[TM][TN][4];
+    if (q.beta != 0.0f) {
+        PreloadTileC<OK, TM, TN>(q, m0 + wm, n0 + wn, lane, cpre);
+    }
+
     uint4 ra[DEPTH][LA::kPerThread];
     uint4 rb[DEPTH][LB::kPerThread];
     const int nk = (q.k + kK - 1) / kK;
@@ -852,18 +909,6 @@ __device__ __forceinline__ void GemmTileDeep(const TileJob& q, int m0, int n0, u
             lb.FetchInto(q.B, q.ldb, q.bK, q.vecB, n0, q.n, kb * kK, q.k, db);
         }
     };
-    dev::StaticFor<0, DEPTH>([&](auto sc) {
-        constexpr int sl = decltype(sc)::value;
-        if (sl < nk) {
-            fetch(sl, ra[sl], rb[sl]);
-        }
-    });
-    if (nk > 0) {
-        la.StoreFastFrom(sA[0], q.aK, ra[0]);
-        lb.StoreFastFrom(sB[0], q.bK, rb[0]);
-    }
-    __syncthreads();
-
     const int fr = lane & 15;
     const int fq = (lane >> 4) * 32;
     uint32_t offA[4], offB[4];
@@ -873,9 +918,49 @@ __device__ __forceinline__ void GemmTileDeep(const TileJob& q, int m0, int n0, u
         offB[i] = SwizzledOffset(wn + i * 16 + fr, fq) - i * 16 * kRowBytes;
     }
 
+    // Interior tile, whole k-blocks, at least 2 DEPTH of them: the refill
+    // groups run without branches, so the compiler can wait for each slot's
+    // own loads (vmcnt counts) instead of for every load in flight.
+    int kbStart = 0;
+    if (interiorA && interiorB && fullK * kK == q.k && nk >= 2 * DEPTH) {
+        dev::StaticFor<0, DEPTH>([&](auto sc) {
+            constexpr int sl = decltype(sc)::value;
+            la.FetchFullInto(sl, ra[sl]);
+            lb.FetchFullInto(sl, rb[sl]);
+        });
+        la.StoreFastFrom(sA[0], q.aK, ra[0]);
+        lb.StoreFastFrom(sB[0], q.bK, rb[0]);
+        __syncthreads();
+        for (; kbStart + 2 * DEPTH <= nk; kbStart += DEPTH) {
+            dev::StaticFor<0, DEPTH>([&](auto sc) {
+                constexpr int sl = decltype(sc)::value;
+                constexpr int nx = (sl + 1) % DEPTH;
+                const int kb = kbStart + sl;
+                la.FetchFullInto(kb + DEPTH, ra[sl]);
+                lb.FetchFullInto(kb + DEPTH, rb[sl]);
+                KBlockMfma<KA, KB, TM, TN>(sA[kb & 1], sB[kb & 1], offA, offB, acc);
+                la.StoreFastFrom(sA[(kb + 1) & 1], q.aK, ra[nx]);
+                lb.StoreFastFrom(sB[(kb + 1) & 1], q.bK, rb[nx]);
+                __syncthreads();
+            });
+        }
+    } else {
+        dev::StaticFor<0, DEPTH>([&](auto sc) {
+            constexpr int sl = decltype(sc)::value;
+            if (sl < nk) {
+                fetch(sl, ra[sl], rb[sl]);
+            }
+        });
+        if (nk > 0) {
+            la.StoreFastFrom(sA[0], q.aK, ra[0]);
+            lb.StoreFastFrom(sB[0], q.bK, rb[0]);
+        }
+        __syncthreads();
+    }
+
     // slot kb % DEPTH holds k-block kb until it is stored to LDS (one
     // iteration before kb is consumed), then takes k-block kb + DEPTH
-    for (int kb0 = 0; kb0 < nk; kb0 += DEPTH) {
+    for (int kb0 = kbStart; kb0 < nk; kb0 += DEPTH) {
         dev::StaticFor<0, DEPTH>([&](auto sc) {
             constexpr int sl = decltype(sc)::value;
             constexpr int nx = (sl + 1) % DEPTH;
@@ -894,7 +979,11 @@ __device__ __forceinline__ void GemmTileDeep(const TileJob& q, int m0, int n0, u
         });
     }
 
-    StoreTileC<OK, TM, TN, false>(q, acc, m0 + wm, n0 + wn, lane);
+    if (q.beta != 0.0f) {
+        StoreTileCPre<OK, TM, TN>(q, acc, cpre, m0 + wm, n0 + wn, lane);
+    } else {
+        StoreTileC<OK, TM, TN, false>(q, acc, m0 + wm, n0 + wn, lane);
+    }
 }
 
 // One problem (or a uniform batch through pointer arrays): one workgroup per

@@ -153,6 +153,16 @@ __device__ __forceinline__ double MaxD(double a, double b) { return b > a ? b : 
 // bounds wherever this is used): the compiler can then issue the loads of a
 // sequence together, where a load under a lane-dependent condition is issued
 // inside a branch and waited for there.
+// The pointer, made opaque to the optimiser at this point: addresses derived
+// from it are computed here instead of being hoisted out of the enclosing
+// refinement loop, where they would stay live across the solves and spill.
+template <typename T>
+__device__ __forceinline__ T* Opaque(T* p)
+{
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 // Keeps the compiler's scheduler from moving instructions across this point:
 // placed after a group of loads, it keeps them issued together ahead of their
 // uses (under register pressure the scheduler otherwise pairs each load with
@@ -188,7 +198,7 @@ __device__ __forceinline__ void FactoredSolve(const MixedSystem& s, RefineShared
     const int wave = t / kWave;
     const int r16 = lane & 15;
     const int grp = lane >> 4;
-    const auto LU = dev::G(s.LU);
+    const auto LU = dev::G(Opaque(s.LU));
     const int blocks = (n + kNb - 1) / kNb;
     for (int p = 0; p < blocks; ++p) {
         const int k0 = p * kNb;
@@ -401,20 +411,24 @@ __device__ __forceinline__ void ResidualColumns(const MixedSystem& s, const Refi
     const int t = threadIdx.x;
     const int myRow = row0 + (t & 3);
     const bool rowOk = myRow < n;
-    const auto A = dev::G(s.A);
+    const auto A = dev::G(Opaque(s.A));
     const size_t lda = static_cast<size_t>(s.lda);
     // eight columns per step, the next eight loaded while these are used
     constexpr int kCols = 8;
     float cur[kCols], nxt[kCols];
+    // a column's base address is uniform (scalar registers), the row a 32-bit
+    // lane offset; columns past n and rows past n read element (0, 0) and are
+    // replaced by 0 at their use
+    const unsigned rowOff = rowOk ? static_cast<unsigned>(myRow) : 0u;
     dev::StaticFor<0, kCols>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
-        cur[j] = LoadAt(A, myRow + static_cast<size_t>(j) * lda, rowOk && j < n);
+        cur[j] = (A + static_cast<size_t>(j < n ? j : 0) * lda)[rowOff];
     });
     for (int j0 = 0; j0 < n; j0 += kCols) {
         dev::StaticFor<0, kCols>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
             const int col = j0 + kCols + j;
-            nxt[j] = LoadAt(A, myRow + static_cast<size_t>(col) * lda, rowOk && col < n);
+            nxt[j] = (A + static_cast<size_t>(col < n ? col : 0) * lda)[rowOff];
         });
         LoadsBeforeUses();
         dev::StaticFor<0, kCols>([&](auto jc) {
@@ -456,7 +470,8 @@ __device__ __forceinline__ double Residual(const MixedSystem& s, RefineShared& s
     const int wave = t / kWave;
     const int q = t & 3;
     const int row0 = (t >> 2) * kResRows;
-    const auto Bm = dev::G(s.B);
+    const int uwave = dev::Uniform(wave);
+    const auto Bm = dev::G(Opaque(s.B));
     const size_t ldb = static_cast<size_t>(s.ldb);
     float bf[kResRows][kResRhs];
     dev::StaticFor<0, kResRhs>([&](auto ec) {
@@ -480,7 +495,7 @@ __device__ __forceinline__ double Residual(const MixedSystem& s, RefineShared& s
         });
     });
     double rs[kResRows] = {0.0, 0.0, 0.0, 0.0};
-    if (wave * kWave < n) {  // the wave's rows 64 wave .. +63 reach into the system
+    if (uwave * kWave < n) {  // the wave's rows 64 wave .. +63 reach into the system
         if (withNorm) {
             ResidualColumns<true>(s, sh, row0, q, r, rs);
         } else {

@@ -59,38 +59,53 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void SolveSmallKernel(
     const bool real = live && r < n;
     const int rc = min(RC, nrhs - c0);
 
+    // Every load below is unconditional, at element 0 of the lane's (or the
+    // first) problem where the element is not needed, and its value is
+    // replaced by the padding afterwards: the loads then issue together
+    // instead of each under a branch with its own wait.
     // factor entries: row r (trans N) or column r (trans T), zero padding
-    const float* A = live ? Aarray[b] : nullptr;
+    const auto A = dev::G(Aarray[live ? b : 0]);
     float a[S];
-    {
-        const int jlim = real ? n : 0;
-        dev::StaticFor<0, S>([&](auto jc) {
-            constexpr int j = decltype(jc)::value;
-            if (j < jlim) {
-                a[j] = TRANS ? A[j + static_cast<size_t>(r) * lda] : A[r + static_cast<size_t>(j) * lda];
-            } else {
-                a[j] = 0.0f;
-            }
-        });
-    }
+    dev::StaticFor<0, S>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const size_t idx = TRANS ? j + static_cast<size_t>(r) * lda : r + static_cast<size_t>(j) * lda;
+        a[j] = A[(real && j < n) ? idx : 0];
+    });
 
     // this lane's own row of B (trans N: permuted below, inside the segment, so
     // the load does not wait for the pivots)
-    float* B = live ? Barray[b] : nullptr;
+    const auto B = dev::G(Barray[live ? b : 0]);
     float x[RC];
     dev::StaticFor<0, RC>([&](auto cc) {
         constexpr int c = decltype(cc)::value;
-        x[c] = (real && c < rc) ? B[r + static_cast<size_t>(c0 + c) * ldb] : 0.0f;
+        x[c] = B[(real && c < rc) ? r + static_cast<size_t>(c0 + c) * ldb : 0];
+    });
+
+    // pivots of the lane's problem
+    int pv[S];
+    if (ipiv != nullptr) {
+        const auto p = dev::G(ipiv + static_cast<size_t>(live ? b : 0) * n);
+        dev::StaticFor<0, S>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            pv[k] = p[k < n ? k : 0];
+        });
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    dev::StaticFor<0, S>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        a[j] = (real && j < n) ? a[j] : 0.0f;
+    });
+    dev::StaticFor<0, RC>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        x[c] = (real && c < rc) ? x[c] : 0.0f;
     });
 
     // row of B that lands at position r after the interchanges
     int src = r;
     if (ipiv != nullptr && real) {
-        const int* p = ipiv + static_cast<size_t>(b) * n;
-        int pv[S];
         dev::StaticFor<0, S>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            pv[k] = k < n ? p[k] - 1 : -1;
+            pv[k] = k < n ? pv[k] - 1 : -1;
         });
         dev::StaticFor<0, S>([&](auto kc) {
             constexpr int k = S - 1 - decltype(kc)::value;

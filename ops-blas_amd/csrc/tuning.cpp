@@ -42,11 +42,11 @@ int GemmBucket(int cls, long long tiles128, int k, int cus)
     return (cls * kGemmSizeBuckets + s) * kGemmKBuckets + kb;
 }
 
-int GroupedBucket(long long tiles128, int cus)
+int GroupedBucket(int cls, long long tiles128, int cus)
 {
-    // < cus, < 4 cus, more
+    // < cus, < 4 cus, more; then the element class
     static const long long kEdges[4] = {4, 16, 0, 0};
-    return SizeBucket(tiles128, cus, kEdges, kGroupedBuckets);
+    return cls * kGroupedSizeBuckets + SizeBucket(tiles128, cus, kEdges, kGroupedSizeBuckets);
 }
 
 int MixedBucket(int cls, long long count, int cus)
@@ -185,7 +185,10 @@ extern "C" aclblasStatus_t aclblasTuningBucket(aclblasHandle_t handle, int op, i
             *bucket = tune::GemmBucket(cls, units, k, cus);
             return ACLBLAS_STATUS_SUCCESS;
         case tune::kOpGrouped:
-            *bucket = tune::GroupedBucket(units, cus);
+            if (cls < 0 || cls > 1) {
+                return ACLBLAS_STATUS_INVALID_VALUE;
+            }
+            *bucket = tune::GroupedBucket(cls, units, cus);
             return ACLBLAS_STATUS_SUCCESS;
         case tune::kOpMixed:
             if (cls < 0 || cls > 1) {

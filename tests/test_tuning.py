@@ -45,8 +45,10 @@ def test_gemm_bucket_boundaries(h):
 
 
 def test_grouped_and_mixed_buckets(h):
-    assert [_bucket(h, B.TUNE_GROUPED, 0, u) for u in (0, CUS - 1, CUS, 4 * CUS - 1, 4 * CUS, 10 ** 9)] == \
-        [0, 0, 1, 1, 2, 2]
+    for cls in (0, 1):
+        assert [_bucket(h, B.TUNE_GROUPED, cls, u) for u in (0, CUS - 1, CUS, 4 * CUS - 1, 4 * CUS, 10 ** 9)] == \
+            [3 * cls + b for b in (0, 0, 1, 1, 2, 2)]
+    assert B.tuning_bucket(h, B.TUNE_GROUPED, 2, 10)[0] == status_code("INVALID_VALUE")
     for cls in (0, 1):
         assert [_bucket(h, B.TUNE_MIXED, cls, u) for u in (1, CUS - 1, CUS, 4 * CUS, 16 * CUS - 1, 16 * CUS)] == \
             [4 * cls + b for b in (0, 0, 1, 2, 2, 3)]
@@ -94,7 +96,8 @@ def test_table_rows_are_legal_and_used(h):
         assert st == 0
         assert (ga, gb) == (a, b), "row %r not honoured" % ((op, bucket, a, b),)
         if op == B.TUNE_GEMM:
-            assert a in (64, 128, 256) and 1 <= b <= 64
+            assert a in (64, 128, 256) or (a in (32, 65) and bucket // 15 == B.CLASS_F32)
+            assert 1 <= b <= 64
             assert not (a == 256 and bucket // 15 == B.CLASS_F32)
         elif op == B.TUNE_GROUPED:
             assert a in (64, 128) and 1 <= b <= 4
