@@ -73,6 +73,10 @@ def parse():
     ap.add_argument("--primary-only", action="store_true", help="skip the comparison and secondary workloads")
     ap.add_argument("--profile-gemm", action="store_true",
                     help="only run three BF16 x BF16 -> FP32 8192^3 GEMMs (for counter collection) and exit")
+    ap.add_argument("--profile-grouped", action="store_true",
+                    help="only run the grouped GEMM workload three times in FP32 and three times in BF16 and exit")
+    ap.add_argument("--profile-sgemm512", action="store_true",
+                    help="only run 200 FP32 512^3 aclblasSgemm calls (NN, alpha 1, beta 0.5, U[-1,1]) and exit")
     ap.add_argument("--profile-fp8", action="store_true",
                     help="only quantise two 8192^2 FP32 operands to E4M3 and run three scaled FP8 8192^3 GEMMs "
                          "(BF16 C) and exit")
@@ -433,7 +437,24 @@ def bench_gemm(torch, B, W, h, timer):
     tf = 2 * m ** 3 / (ms * 1e-3) / 1e12
     out["fp32_8192"] = {"ms": ms, "tflops": tf, "frac_of_peak": tf / FP32_PEAK_TFLOPS}
     del a, b, c
-    # grouped FP32: 64 groups, m, n, k in {64, 80, ..., 2048}, seed 0, one problem per group
+    f32_call, bf16_call, flops, keep = grouped_legs(torch, B, h, dev)
+    f32_call()
+    ms = timer(f32_call, 5)
+    tf = flops / (ms * 1e-3) / 1e12
+    out["grouped_fp32_64"] = {"ms": ms, "tflops": tf, "frac_of_peak": tf / FP32_PEAK_TFLOPS, "flop": flops}
+    assert bf16_call() == 0
+    ms = timer(bf16_call, 5)
+    tf = flops / (ms * 1e-3) / 1e12
+    out["grouped_bf16_ex_64"] = {"ms": ms, "tflops": tf, "frac_of_peak": tf / BF16_PEAK_TFLOPS, "flop": flops}
+    del keep
+    return out
+
+
+def grouped_legs(torch, B, h, dev):
+    """The grouped workload: 64 groups of one problem, m, n, k in {64, 80, ..., 2048} (numpy seed 0), alpha 1,
+    beta 0.5.  Returns the FP32 aclblasSgemmGroupedBatched call, the BF16 (BF16 C) aclblasGemmGroupedBatchedEx
+    call on the same shapes, the flop count and the tensors to keep alive."""
+    import numpy as np
     rng = np.random.default_rng(0)
     mnk = rng.integers(0, 125, size=(64, 3)) * 16 + 64
     As, Bs, Cs = [], [], []
@@ -444,29 +465,20 @@ def bench_gemm(torch, B, W, h, timer):
         Cs.append(torch.rand(int(mm * nn), device=dev))
         flops += 2 * int(mm) * int(nn) * int(kk)
     g = [int(x) for x in mnk[:, 0]], [int(x) for x in mnk[:, 1]], [int(x) for x in mnk[:, 2]]
-    f = lambda: B.sgemm_grouped_batched(h.value, [B.OP_N] * 64, [B.OP_N] * 64, g[0], g[1], g[2], [1.0] * 64,
-                                        [x.data_ptr() for x in As], g[0], [x.data_ptr() for x in Bs], g[2],
-                                        [0.5] * 64, [x.data_ptr() for x in Cs], g[0], [1] * 64)
-    f()
-    ms = timer(f, 5)
-    tf = flops / (ms * 1e-3) / 1e12
-    out["grouped_fp32_64"] = {"ms": ms, "tflops": tf, "frac_of_peak": tf / FP32_PEAK_TFLOPS, "flop": flops}
-    # the same 64 shapes in BF16 (BF16 C) through aclblasGemmGroupedBatchedEx
-    del As, Bs, Cs
+    f32_call = lambda: B.sgemm_grouped_batched(h.value, [B.OP_N] * 64, [B.OP_N] * 64, g[0], g[1], g[2],
+                                               [1.0] * 64, [x.data_ptr() for x in As], g[0],
+                                               [x.data_ptr() for x in Bs], g[2], [0.5] * 64,
+                                               [x.data_ptr() for x in Cs], g[0], [1] * 64)
     Ah = [torch.randn(int(mm * kk), device=dev).to(torch.bfloat16) for mm, nn, kk in mnk]
     Bh = [torch.randn(int(kk * nn), device=dev).to(torch.bfloat16) for mm, nn, kk in mnk]
     Ch = [torch.randn(int(mm * nn), device=dev).to(torch.bfloat16) for mm, nn, kk in mnk]
     ptr = lambda ts: torch.tensor([x.data_ptr() for x in ts], dtype=torch.int64, device=dev)
     pa, pb, pc = ptr(Ah), ptr(Bh), ptr(Ch)
-    f = lambda: B.gemm_grouped_batched_ex(h.value, [B.OP_N] * 64, [B.OP_N] * 64, g[0], g[1], g[2], [1.0] * 64,
-                                          pa.data_ptr(), B.ACL_BF16, g[0], pb.data_ptr(), B.ACL_BF16, g[2],
-                                          [0.5] * 64, pc.data_ptr(), B.ACL_BF16, g[0], [1] * 64,
-                                          B.COMPUTE["COMPUTE_32F"])
-    assert f() == 0
-    ms = timer(f, 5)
-    tf = flops / (ms * 1e-3) / 1e12
-    out["grouped_bf16_ex_64"] = {"ms": ms, "tflops": tf, "frac_of_peak": tf / BF16_PEAK_TFLOPS, "flop": flops}
-    return out
+    bf16_call = lambda: B.gemm_grouped_batched_ex(h.value, [B.OP_N] * 64, [B.OP_N] * 64, g[0], g[1], g[2],
+                                                  [1.0] * 64, pa.data_ptr(), B.ACL_BF16, g[0], pb.data_ptr(),
+                                                  B.ACL_BF16, g[2], [0.5] * 64, pc.data_ptr(), B.ACL_BF16, g[0],
+                                                  [1] * 64, B.COMPUTE["COMPUTE_32F"])
+    return f32_call, bf16_call, flops, (As, Bs, Cs, Ah, Bh, Ch, pa, pb, pc)
 
 
 def bench_adjoint(torch, B, W, h, timer):
@@ -585,6 +597,31 @@ def main():
         h.destroy()
         print("fp8 profile run done")
         return
+    if args.profile_grouped:
+        f32_call, bf16_call, _, keep = grouped_legs(torch, B, h, dev)
+        for _ in range(3):
+            assert f32_call() == 0
+        for _ in range(3):
+            assert bf16_call() == 0
+        torch.cuda.synchronize()
+        del keep
+        h.destroy()
+        print("grouped profile run done")
+        return
+
+    if args.profile_sgemm512:
+        m = 512
+        a = (torch.rand(m * m, device=dev) * 2 - 1)
+        b = (torch.rand(m * m, device=dev) * 2 - 1)
+        c = (torch.rand(m * m, device=dev) * 2 - 1)
+        for _ in range(200):
+            st = B.sgemm(h.value, B.OP_N, B.OP_N, m, m, m, 1.0, a.data_ptr(), m, b.data_ptr(), m, 0.5, c.data_ptr(), m)
+            assert st == 0, st
+        torch.cuda.synchronize()
+        h.destroy()
+        print("sgemm 512 profile run done")
+        return
+
     if args.profile_gemm:
         m = 8192
         a = torch.randn(m * m, device=dev).to(torch.bfloat16)
