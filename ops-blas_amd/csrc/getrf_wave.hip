@@ -59,13 +59,29 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void GetrfWaveKernel(
     // global address space: plain global loads and stores instead of flat ones
     typedef __attribute__((address_space(1))) float GlobalFloat;
     GlobalFloat* A = live ? (GlobalFloat*)(Aarray[b]) : nullptr;
+    // Every load is unconditional (element 0 of the first matrix where the
+    // element is padding) and the padding is applied afterwards: a load under
+    // a lane-dependent condition is issued inside a branch and waited for
+    // there, which made the S loads of a row one round trip each.
+    float a0[R][S];
+    {
+        GlobalFloat* A0 = (GlobalFloat*)(Aarray[live ? b : 0]);
+        dev::StaticFor<0, R>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            dev::StaticFor<0, S>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                const bool need = real[i] && j < n;
+                a0[i][j] = A0[need ? r + i * L + static_cast<size_t>(j) * lda : 0];
+            });
+        });
+    }
+    __builtin_amdgcn_sched_barrier(0);
     float a[R][S];
     dev::StaticFor<0, R>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
-        const int jlim = real[i] ? n : 0;
         dev::StaticFor<0, S>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
-            a[i][j] = j < jlim ? A[r + i * L + static_cast<size_t>(j) * lda] : 0.0f;
+            a[i][j] = (real[i] && j < n) ? a0[i][j] : 0.0f;
         });
     });
 

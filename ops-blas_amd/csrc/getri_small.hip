@@ -53,18 +53,30 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock<S>) void GetriColumnKernel(
     float* t = static_cast<float*>(__builtin_assume_aligned(lds[wave], 16));
 
     // factors, zero-padded, column-major with leading dimension S
-    const float* A = Aarray[b];
-    for (int j = 0; j < S; ++j) {
-        if (lane < S) {
-            t[j * S + lane] = (lane < n && j < n) ? A[lane + static_cast<size_t>(j) * lda] : 0.0f;
-        }
+    // (all loads unconditional, at element 0 where the entry is padding, so
+    // that they issue together; the padding is applied when storing to LDS)
+    {
+        const auto A = dev::G(Aarray[b]);
+        float v[S];
+        dev::StaticFor<0, S>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            v[j] = A[(lane < n && j < n) ? lane + static_cast<size_t>(j) * lda : 0];
+        });
+        __builtin_amdgcn_sched_barrier(0);
+        dev::StaticFor<0, S>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            if (lane < S) {
+                t[j * S + lane] = (lane < n && j < n) ? v[j] : 0.0f;
+            }
+        });
     }
 
     // row at which column c's unit entry lands after the interchanges k <-> piv[k] - 1,
     // applied for k = 0 .. n-1 (entries outside [1, n] skipped)
     int pos = lane;
     if (ipiv != nullptr) {
-        const int myp = lane < n ? ipiv[static_cast<size_t>(b) * n + lane] - 1 : -1;
+        const int pv = dev::G(ipiv)[static_cast<size_t>(b) * n + (lane < n ? lane : 0)];
+        const int myp = lane < n ? pv - 1 : -1;
         for (int k = 0; k < n; ++k) {
             const int pk = __builtin_amdgcn_readlane(myp, k);
             if (pk >= 0 && pk < n && pk != k) {
