@@ -28,6 +28,7 @@ namespace {
 using dev::kWave;
 
 constexpr int kBlk = 64;            // rows per diagonal block
+constexpr int kChunk = 8;           // entries of a trailing row loaded together
 constexpr int kMaxRc = 8;           // right-hand sides per workgroup
 constexpr size_t kLdsBudget = 64 * 1024;
 
@@ -84,7 +85,7 @@ __device__ void BuildPerm(int* perm, int* scratch, const int* piv, int n)
 // UNIT), in 64-row blocks.  Tel(i, j) returns T[i][j] for i > j (and i == j
 // when !UNIT).  bs is the LDS chunk, row-major [n][RC].
 template <int RC, bool UNIT, typename Tel>
-__device__ void LowerSolve(float* bs, int n, Tel tel)
+__device__ __forceinline__ void LowerSolve(float* bs, int n, Tel tel)
 {
     const int t = threadIdx.x;
     const int lane = dev::LaneId();
@@ -94,9 +95,18 @@ __device__ void LowerSolve(float* bs, int n, Tel tel)
             float tr[kBlk];
             float x[RC];
             const bool valid = lane < nb;
+            // unconditional loads (the block's first diagonal element where
+            // the entry is not needed), padding afterwards: the loads issue
+            // together instead of one branch-and-wait each
 #pragma unroll
             for (int jj = 0; jj < kBlk; ++jj) {
-                tr[jj] = (valid && jj <= lane && jj < nb) ? tel(j0 + lane, j0 + jj) : 0.0f;
+                const bool need = valid && jj <= lane && jj < nb;
+                tr[jj] = tel(need ? j0 + lane : j0, need ? j0 + jj : j0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int jj = 0; jj < kBlk; ++jj) {
+                tr[jj] = (valid && jj <= lane && jj < nb) ? tr[jj] : 0.0f;
             }
 #pragma unroll
             for (int r = 0; r < RC; ++r) {
@@ -131,11 +141,21 @@ __device__ void LowerSolve(float* bs, int n, Tel tel)
             for (int r = 0; r < RC; ++r) {
                 acc[r] = bs[i * RC + r];
             }
-            for (int jj = 0; jj < nb; ++jj) {
-                const float tv = tel(i, j0 + jj);
+            // eight entries of the row per batch of loads, FMAs in column order
+            for (int jc = 0; jc < nb; jc += kChunk) {
+                float tv[kChunk];
 #pragma unroll
-                for (int r = 0; r < RC; ++r) {
-                    acc[r] = fmaf(-tv, bs[(j0 + jj) * RC + r], acc[r]);
+                for (int e = 0; e < kChunk; ++e) {
+                    tv[e] = tel(i, j0 + (jc + e < nb ? jc + e : 0));
+                }
+#pragma unroll
+                for (int e = 0; e < kChunk; ++e) {
+                    if (jc + e < nb) {
+#pragma unroll
+                        for (int r = 0; r < RC; ++r) {
+                            acc[r] = fmaf(-tv[e], bs[(j0 + jc + e) * RC + r], acc[r]);
+                        }
+                    }
                 }
             }
 #pragma unroll
@@ -150,7 +170,7 @@ __device__ void LowerSolve(float* bs, int n, Tel tel)
 // Backward substitution with an upper-triangular factor T (unit diagonal when
 // UNIT): rows processed from the bottom, x_j applied in decreasing j.
 template <int RC, bool UNIT, typename Tel>
-__device__ void UpperSolve(float* bs, int n, Tel tel)
+__device__ __forceinline__ void UpperSolve(float* bs, int n, Tel tel)
 {
     const int t = threadIdx.x;
     const int lane = dev::LaneId();
@@ -163,7 +183,13 @@ __device__ void UpperSolve(float* bs, int n, Tel tel)
             const bool valid = lane < nb;
 #pragma unroll
             for (int jj = 0; jj < kBlk; ++jj) {
-                tr[jj] = (valid && jj >= lane && jj < nb) ? tel(j0 + lane, j0 + jj) : 0.0f;
+                const bool need = valid && jj >= lane && jj < nb;
+                tr[jj] = tel(need ? j0 + lane : j0, need ? j0 + jj : j0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int jj = 0; jj < kBlk; ++jj) {
+                tr[jj] = (valid && jj >= lane && jj < nb) ? tr[jj] : 0.0f;
             }
 #pragma unroll
             for (int r = 0; r < RC; ++r) {
@@ -198,11 +224,21 @@ __device__ void UpperSolve(float* bs, int n, Tel tel)
             for (int r = 0; r < RC; ++r) {
                 acc[r] = bs[i * RC + r];
             }
-            for (int jj = nb - 1; jj >= 0; --jj) {
-                const float tv = tel(i, j0 + jj);
+            // eight entries per batch of loads, FMAs in decreasing column order
+            for (int jc = nb - 1; jc >= 0; jc -= kChunk) {
+                float tv[kChunk];
 #pragma unroll
-                for (int r = 0; r < RC; ++r) {
-                    acc[r] = fmaf(-tv, bs[(j0 + jj) * RC + r], acc[r]);
+                for (int e = 0; e < kChunk; ++e) {
+                    tv[e] = tel(i, j0 + (jc - e >= 0 ? jc - e : 0));
+                }
+#pragma unroll
+                for (int e = 0; e < kChunk; ++e) {
+                    if (jc - e >= 0) {
+#pragma unroll
+                        for (int r = 0; r < RC; ++r) {
+                            acc[r] = fmaf(-tv[e], bs[(j0 + jc - e) * RC + r], acc[r]);
+                        }
+                    }
                 }
             }
 #pragma unroll
@@ -231,8 +267,8 @@ __global__ __launch_bounds__(256) void SolveKernel(
     const int chunk = blockIdx.x - b * chunks;
     const int r0 = chunk * RC;
     const int rc = min(RC, nrhs - r0);
-    const float* A = Aarray[b];
-    float* B = Barray[b];
+    const auto A = dev::G(Aarray[b]);
+    const auto B = dev::G(Barray[b]);
     const int t = threadIdx.x;
 
     if (ipiv != nullptr) {
