@@ -92,6 +92,7 @@ struct FactorShared {
     alignas(16) uint64_t cand64[2][kWaves];               // its packed key (PackCand), 0: no candidate
     int rowmap[kMaxN];                                    // panel row map (position -> row, relative)
     int pw[kNb];                                          // pivot position chosen at each panel column
+    int zero;                                             // zero-pivot column of a one-wave panel
     alignas(16) float linv[kNb][kLd];                     // inv(L11), row-major
     // L11 lives from the panel's write-back to its inverse, the sweep's
     // per-wave tiles only during the sweep: they share the space
@@ -163,10 +164,16 @@ __device__ __forceinline__ int FactorPanel(const PanelView& pv, int k0, int* gma
     }
     int pos = r;
     int zeroCol = 0;
+    // A panel of at most 64 rows lives in wave 0 alone: the other waves skip
+    // the column loop and wave 0 exchanges its pivot rows with wave-level
+    // synchronisation instead of workgroup barriers (same pivots, same
+    // arithmetic); the zero-pivot column is shared through LDS afterwards.
+    const bool solo = rows <= kWave;  // uniform over the workgroup
 
     // Straight-line per column (no early exit, so the row stays in the same
     // registers): a column past the panel's width or after a zero pivot is
     // skipped as a whole by every wave alike.
+    if (!solo || wave == 0) {
     dev::StaticFor<0, kNb>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
         constexpr int buf = j & 1;
@@ -196,19 +203,26 @@ __device__ __forceinline__ int FactorPanel(const PanelView& pv, int k0, int* gma
             }
         }
         if (live) {
-            __syncthreads();
+            if (solo) {
+                WaveSync();
+            } else {
+                __syncthreads();
+            }
         }
         float pivot = 0.0f;
         int pw = 0;
         float u[kNb];
         if (live) {
-            // the winner over the waves: the largest packed candidate
+            // the winner over the waves: the largest packed candidate (wave 0's
+            // own for a one-wave panel)
             uint64_t best4 = sh.cand64[buf][0];
-            dev::StaticFor<1, kWaves>([&](auto vc) {
-                constexpr int v = decltype(vc)::value;
-                const uint64_t c = sh.cand64[buf][v];
-                best4 = c > best4 ? c : best4;
-            });
+            if (!solo) {
+                dev::StaticFor<1, kWaves>([&](auto vc) {
+                    constexpr int v = decltype(vc)::value;
+                    const uint64_t c = sh.cand64[buf][v];
+                    best4 = c > best4 ? c : best4;
+                });
+            }
             const uint32_t lo = static_cast<uint32_t>(best4);
             const int wb = dev::Uniform(static_cast<int>(lo & 0xFFu));
             pw = dev::Uniform(0xFFFF - static_cast<int>((lo >> 8) & 0xFFFFu));
@@ -253,7 +267,20 @@ __device__ __forceinline__ int FactorPanel(const PanelView& pv, int k0, int* gma
                 });
             }
         }
+        if (solo) {
+            // the next column's candidate slot is rewritten only after every
+            // lane of the wave has read this one
+            WaveSync();
+        }
     });
+    }
+    if (solo) {
+        if (threadIdx.x == 0) {
+            sh.zero = zeroCol;
+        }
+        __syncthreads();
+        zeroCol = sh.zero;
+    }
     const int done = zeroCol != 0 ? zeroCol - 1 - k0 : w;
     *wl = done;
 
