@@ -501,6 +501,8 @@ __device__ __forceinline__ void UpdateTile(const PanelView& pv, int k0, int c0, 
                     constexpr int q = decltype(pc)::value;
                     la[q] = *reinterpret_cast<const bf16x8*>(&sh.op.l[q][lr][kq]);
                 });
+                // every plane in its own registers before the products start
+                __builtin_amdgcn_sched_barrier(0);
                 f32x4 v = acc[t];
                 if constexpr (P == 3) {
                     v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ub[0], la[2], v, 0, 0, 0);
